@@ -1,0 +1,154 @@
+// Closed-form symmetric eigenvalues and the Frangi vesselness formula, as
+// device functions shared by the fused Hessian epilogue and the standalone
+// kernels.
+//
+// Reference: feature/corner.py:473-696 (eigen kernels), filters/ridges.py:
+// 228-319 (Frangi inner kernels), paths relative to
+// python/cucim/src/cucim/skimage/.  Transcendentals are HIP's device math
+// library (OCML) functions.
+#pragma once
+
+#include "hc_common.h"
+
+namespace hc {
+
+template <typename F>
+__device__ __forceinline__ void swapv(F& x, F& y) {
+  F t = y;
+  y = x;
+  x = t;
+}
+
+// 2x2: the temporaries tmp1/tmp2 are of the element type F (corner.py:475-490),
+// so for float32 input the arithmetic after each double product is float32.
+template <typename F>
+__device__ __forceinline__ void eig22(F M00, F M01, F M11, int descending, int abs_sort, F& l1,
+                                      F& l2) {
+  const double m00 = (double)M00, m01 = (double)M01, m11 = (double)M11;
+  F tmp1 = (F)(m01 * m01);
+  tmp1 *= (F)4;
+  F tmp2 = (F)(m00 - m11);
+  tmp2 *= tmp2;
+  tmp2 += tmp1;
+  tmp2 = sqrt(tmp2);
+  tmp2 /= (F)2;
+  tmp1 = (F)(m00 + m11);
+  tmp1 /= (F)2;
+  if (!descending) {
+    l1 = tmp1 - tmp2;
+    l2 = tmp1 + tmp2;
+    if (abs_sort && fabs(l1) > fabs(l2)) swapv(l1, l2);
+  } else {
+    l1 = tmp1 + tmp2;
+    l2 = tmp1 - tmp2;
+    if (abs_sort && fabs(l1) < fabs(l2)) swapv(l1, l2);
+  }
+}
+
+// 3x3, Deledalle et al. (2017) closed form in double (corner.py:566-612) for
+// the matrix [[a, d, f], [d, b, e], [f, e, c]], then the reference's three
+// compare-swap steps (corner.py:613-647).  With abs_sort the comparison keys
+// are the absolute values taken before any swap and are not updated by the
+// swaps; this ordering is reproduced as is.
+template <typename F>
+__device__ __forceinline__ void eig33(F aa, F bb, F cc, F dd, F ee, F ff, int descending,
+                                      int abs_sort, F& l1, F& l2, F& l3) {
+  const double a = aa, b = bb, c = cc, d = dd, e = ee, f = ff;
+  const double d_sq = d * d, e_sq = e * e, f_sq = f * f;
+  const double tmpa = (2 * a - b - c);
+  const double tmpb = (2 * b - a - c);
+  const double tmpc = (2 * c - a - b);
+  double x2 = -tmpa * tmpb * tmpc;
+  x2 += 9 * (tmpc * d_sq + tmpb * f_sq + tmpa * e_sq);
+  x2 -= 54 * (d * e * f);
+  double x1 = a * a + b * b + c * c - a * b - a * c - b * c + 3 * (d_sq + e_sq + f_sq);
+  x1 = fmax(x1, 0.0);
+  double phi;
+  if (x2 == 0.0) {
+    phi = M_PI / 2.0;
+  } else {
+    const double arg = fmax(4 * x1 * x1 * x1 - x2 * x2, 0.0);
+    phi = atan(sqrt(arg) / x2);
+    if (x2 < 0) phi += M_PI;
+  }
+  const double x1_term = (2.0 / 3.0) * sqrt(x1);
+  const double abc = (a + b + c) / 3.0;
+  F lam1 = (F)(abc - x1_term * cos(phi / 3.0));
+  F lam2 = (F)(abc + x1_term * cos((phi - M_PI) / 3.0));
+  F lam3 = (F)(abc + x1_term * cos((phi + M_PI) / 3.0));
+  // the network runs on (var1, lam2, var3) = (lam1, lam2, lam3) ascending,
+  // (lam3, lam2, lam1) descending
+  F& v1 = descending ? lam3 : lam1;
+  F& v3 = descending ? lam1 : lam3;
+  if (abs_sort) {
+    const F k1 = fabs(v1), k2 = fabs(lam2), k3 = fabs(v3);
+    if (k1 > k2) swapv(v1, lam2);
+    if (k1 > k3) swapv(v1, v3);
+    if (k2 > k3) swapv(lam2, v3);
+  } else {
+    if (v1 > lam2) swapv(v1, lam2);
+    if (v1 > v3) swapv(v1, v3);
+    if (lam2 > v3) swapv(lam2, v3);
+  }
+  l1 = lam1;
+  l2 = lam2;
+  l3 = lam3;
+}
+
+// r_g = sum lambda^2 in F (ridges.py:228-240, 270-283)
+template <typename F>
+__device__ __forceinline__ F frangi_rg2(F l1, F l2) {
+  F r = l1 * l1;
+  return fma_(l2, l2, r);
+}
+template <typename F>
+__device__ __forceinline__ F frangi_rg3(F l1, F l2, F l3) {
+  F r = l1 * l1;
+  r = fma_(l2, l2, r);
+  return fma_(l3, l3, r);
+}
+
+// ridges.py:243-267: eigenvalues ascending by |value|
+template <typename F>
+__device__ __forceinline__ F frangi_v2(F l1, F l2, F r_g, double beta_sq, double gamma_sq) {
+  F r_b = fabs(l1) / fmax(l2, (F)1.0e-10);
+  r_b *= r_b;
+  F result = (F)exp(-(double)r_b / beta_sq);
+  result = (F)((double)result * (1.0 - exp(-(double)r_g / gamma_sq)));
+  return result;
+}
+
+// ridges.py:286-319
+template <typename F>
+__device__ __forceinline__ F frangi_v3(F l1, F l2, F l3, F r_g, double alpha_sq, double beta_sq,
+                                       double gamma_sq) {
+  const F lam2 = fmax(l2, (F)1.0e-10);
+  const F lam3 = fmax(l3, (F)1.0e-10);
+  F r_a = lam2 / lam3;
+  r_a *= r_a;
+  const F r_b = (l1 * l1) / (lam2 * lam3);
+  F result = (F)(1.0 - exp(-(double)r_a / alpha_sq));
+  result = (F)((double)result * exp(-(double)r_b / beta_sq));
+  result = (F)((double)result * (1.0 - exp(-(double)r_g / gamma_sq)));
+  return result;
+}
+
+// cp.maximum: NaN-propagating
+template <typename F>
+__device__ __forceinline__ F nanmax(F a, F b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  return a > b ? a : b;
+}
+
+// gamma from the first sigma (ridges.py:447-453), in double as the host
+// Python computes it: gamma = sqrt(s)/2; gamma_sq = 2.0 if s == 0 else
+// max(2*gamma*gamma, 1e-10) (Python's max keeps a NaN first argument).
+__device__ __forceinline__ double gamma_sq_from_max(double s_max) {
+  const double g = sqrt(s_max) / 2.0;
+  if (s_max == 0.0) return 2.0;
+  const double g2 = 2 * g * g;
+  return (1e-10 > g2) ? 1e-10 : g2;
+}
+
+}  // namespace hc

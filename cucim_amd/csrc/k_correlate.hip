@@ -1,0 +1,507 @@
+// correlate1d / correlate / fused 2-D separable kernels for gfx950.
+//
+// Reference behaviour (paths relative to python/cucim/src/cucim/skimage/):
+//   _vendored/_ndimage_filters.py:271-344   1-D dispatch (shared_memory vs elementwise)
+//   _vendored/_ndimage_filters_core.py:247-408  elementwise n-D kernel
+//   filters/_separable_filtering.py:977-1070    shared-memory line kernel
+//   _vendored/_ndimage_util.py:186-308       boundary modes
+//
+// Kernels:
+//   k_gather         one work-item per output, any strides, any kernel length
+//                    (the 'elementwise' algorithm and the universal fallback)
+//   k_tiled_contig   axis is the contiguous one: each wave stages a
+//                    1024-output line segment plus halo in LDS and computes
+//                    4 outputs per lane from 16-byte LDS reads
+//   k_tiled_strided  axis is strided: a work-group stages 64 lines x
+//                    (64 outputs + halo) in LDS; each lane computes 16
+//                    consecutive outputs of its line
+//   k_sep2d          two passes of a 2-D array in one launch through an LDS
+//                    tile; the intermediate is rounded to the output dtype
+#include <algorithm>
+#include <cstring>
+
+#include "hc_conv.h"
+
+namespace hc {
+
+// ------------------------------------------------------------ gather --
+template <typename W>
+struct GatherArgs {
+  const void* in;
+  void* out;
+  int in_dt, out_dt;
+  int ndim;
+  int all_const;
+  int64_t total;
+  int64_t shape[HC_MAX_NDIM];
+  int64_t istride[HC_MAX_NDIM];
+  int32_t wsh[HC_MAX_NDIM];   // footprint per dim (1 = not filtered)
+  int32_t off[HC_MAX_NDIM];   // wsh//2 + origin
+  int32_t mode[HC_MAX_NDIM];
+  int64_t ntaps;
+  const W* wdev;  // taps when ntaps exceeds the inline capacity
+  W cval;
+  TapPack<W> tp;
+};
+
+template <typename W>
+__global__ __launch_bounds__(256) void k_gather(const GatherArgs<W> a) {
+  typedef const __attribute__((address_space(4))) W* cW;
+  const cW wd = (cW)a.wdev;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < a.total;
+       i += (int64_t)gridDim.x * 256) {
+    int64_t pos[HC_MAX_NDIM];
+    int64_t rem = i;
+#pragma unroll
+    for (int d = HC_MAX_NDIM - 1; d >= 0; --d) {
+      pos[d] = 0;
+      if (d < a.ndim) {
+        const int64_t s = a.shape[d];
+        pos[d] = rem % s;
+        rem /= s;
+      }
+    }
+    int32_t k[HC_MAX_NDIM];
+#pragma unroll
+    for (int d = 0; d < HC_MAX_NDIM; ++d) k[d] = 0;
+    W acc = (W)0;
+    for (int64_t t = 0; t < a.ntaps; ++t) {
+      const W w = wd ? wd[t] : a.tp.w[t];
+      if (nz(w)) {
+        int64_t offs = 0;
+        bool outside = false;
+#pragma unroll
+        for (int d = 0; d < HC_MAX_NDIM; ++d) {
+          if (d < a.ndim) {
+            int64_t q = pos[d] + k[d] - a.off[d];
+            q = remap(q, a.shape[d], a.mode[d]);
+            if (q < 0) { outside = true; q = 0; }
+            offs += q * a.istride[d];
+          }
+        }
+        const W v = outside ? a.cval : load_as<W>(a.in, offs, a.in_dt);
+        acc = fma_(v, w, acc);
+      }
+      // advance the C-order tap counter
+#pragma unroll
+      for (int d = HC_MAX_NDIM - 1; d >= 0; --d) {
+        if (d < a.ndim) {
+          if (++k[d] < a.wsh[d]) break;
+          k[d] = 0;
+        }
+      }
+    }
+    store_cast<W>(a.out, i, a.out_dt, acc);
+  }
+}
+
+// ------------------------------------------------------ tiled kernels --
+template <typename W>
+struct TiledArgs {
+  const void* in;
+  void* out;
+  int in_dt, out_dt;
+  int64_t nlines;  // number of lines
+  int64_t N;       // line length
+  int64_t inner;   // element stride along a line (1 = contiguous)
+  int64_t nseg;    // segments per line
+  int mode;
+  W cval;
+  TapPack<W> tp;   // K padded to a multiple of 4
+};
+
+constexpr int kSeg = 1024;       // contiguous: outputs per wave segment
+constexpr int kStrCols = 64;     // strided: lines per work-group
+constexpr int kStrStride = 65;   // strided: LDS row stride (elements)
+constexpr int kStrTA = 64;       // strided: outputs per line per work-group
+
+// Spread consecutive work-groups of a launch over the 8 XCDs in contiguous
+// chunks, so tiles that share halo rows sit behind the same L2.
+__device__ __forceinline__ int64_t xcd_swizzle(int64_t b, int64_t nb) {
+  const int64_t nx = 8;
+  if (nb < 2 * nx) return b;
+  const int64_t per = nb / nx;        // full rounds
+  const int64_t full = per * nx;
+  if (b >= full) return b;
+  return (b % nx) * per + b / nx;
+}
+
+template <typename W>
+__global__ __launch_bounds__(256) void k_tiled_contig(const TiledArgs<W> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  W* smem = reinterpret_cast<W*>(smem_raw);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int L = kSeg + a.tp.K + 4;  // conv_row4 reads 8 past the last group
+  W* ext = smem + wv * L;
+  const int64_t nb = gridDim.x;
+  const int64_t seg = xcd_swizzle(blockIdx.x, nb) * 4 + wv;
+  const bool active = seg < a.nlines * a.nseg;
+  int64_t lbase = 0, s0 = 0;
+  if (active) {
+    const int64_t line = seg / a.nseg;
+    s0 = (seg - line * a.nseg) * kSeg;
+    lbase = line * a.N;
+    const int64_t p0 = s0 - a.tp.anchor;
+    const bool interior = p0 >= 0 && p0 + L <= a.N;
+    if (interior) {
+      for (int q = lane; q < L; q += 64) ext[q] = load_as<W>(a.in, lbase + p0 + q, a.in_dt);
+    } else {
+      for (int q = lane; q < L; q += 64) {
+        const int64_t r = remap(p0 + q, a.N, a.mode);
+        ext[q] = r < 0 ? a.cval : load_as<W>(a.in, lbase + r, a.in_dt);
+      }
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+#pragma unroll 1
+  for (int g = 0; g < kSeg / 256; ++g) {
+    const int o0 = 256 * g + 4 * lane;
+    const int64_t o = s0 + o0;
+    if (o >= a.N) break;
+    W acc[4] = {(W)0, (W)0, (W)0, (W)0};
+    conv_row4<W>(acc, ext + o0, a.tp);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (o + j < a.N) store_cast<W>(a.out, lbase + o + j, a.out_dt, acc[j]);
+  }
+}
+
+template <typename W>
+__global__ __launch_bounds__(256) void k_tiled_strided(const TiledArgs<W> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  W* ext = reinterpret_cast<W*>(smem_raw);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t nlb = (a.nlines + kStrCols - 1) / kStrCols;
+  const int64_t tile = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int64_t lb = tile % nlb, sg = tile / nlb;
+  const int64_t L = lb * kStrCols + lane;
+  const bool lok = L < a.nlines;
+  const int64_t lo = lok ? L / a.inner : 0, li = lok ? L - lo * a.inner : 0;
+  const int64_t lbase = lo * a.N * a.inner + li;
+  const int64_t a0 = sg * kStrTA;
+  const int R = kStrTA + a.tp.K;
+  const int64_t p0 = a0 - a.tp.anchor;
+  const bool interior = p0 >= 0 && p0 + R <= a.N;
+  for (int q = wv; q < R; q += 4) {
+    W v = a.cval;
+    if (lok) {
+      if (interior) {
+        v = load_as<W>(a.in, lbase + (p0 + q) * a.inner, a.in_dt);
+      } else {
+        const int64_t r = remap(p0 + q, a.N, a.mode);
+        if (r >= 0) v = load_as<W>(a.in, lbase + r * a.inner, a.in_dt);
+      }
+    }
+    ext[q * kStrStride + lane] = v;
+  }
+  __syncthreads();
+  if (!lok) return;
+  const int64_t n0 = a0 + 16 * wv;
+  if (n0 >= a.N) return;
+  W acc[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) acc[j] = (W)0;
+  conv_col16<W, kStrStride>(acc, ext + (16 * wv) * kStrStride + lane, a.tp);
+#pragma unroll
+  for (int j = 0; j < 16; ++j)
+    if (n0 + j < a.N) store_cast<W>(a.out, lbase + (n0 + j) * a.inner, a.out_dt, acc[j]);
+}
+
+// ------------------------------------------------------ fused 2-D --
+constexpr int kSepCols = 128;  // LDS tile width (elements)
+
+template <typename W>
+struct Sep2dArgs {
+  const void* in;
+  void* out;
+  int in_dt, out_dt;
+  int64_t H, Wd;
+  int TX;
+  int64_t ntx, ntiles;
+  int mode0, mode1;
+  W cval;
+  TapPack<W> t0, t1;
+};
+
+template <typename W, int TY>
+__global__ __launch_bounds__(256) void k_sep2d(const Sep2dArgs<W> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  W* A = reinterpret_cast<W*>(smem_raw);
+  const int RY = TY + a.t0.K;
+  W* U = A + RY * kSepCols;
+  const int CX = a.TX + a.t1.K;
+  const int nq = a.TX / 4;
+  for (int64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    const int64_t ty = tile / a.ntx, tx = tile - ty * a.ntx;
+    const int64_t y0 = ty * TY, x0 = tx * a.TX;
+    // stage the input tile with both axes boundary-extended
+    for (int e = threadIdx.x; e < RY * kSepCols; e += 256) {
+      const int q = e / kSepCols, c = e - q * kSepCols;
+      if (c >= CX) continue;
+      const int64_t y = remap(y0 - a.t0.anchor + q, a.H, a.mode0);
+      const int64_t x = remap(x0 - a.t1.anchor + c, a.Wd, a.mode1);
+      A[e] = (y < 0 || x < 0) ? a.cval : load_as<W>(a.in, y * a.Wd + x, a.in_dt);
+    }
+    __syncthreads();
+    // pass 1 (axis 0) over every staged column, rounded to the output dtype
+    for (int it = threadIdx.x; it < (TY / 16) * kSepCols; it += 256) {
+      const int ch = it / kSepCols, c = it - ch * kSepCols;
+      if (c >= CX) continue;
+      W acc[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) acc[j] = (W)0;
+      conv_col16<W, kSepCols>(acc, A + (16 * ch) * kSepCols + c, a.t0);
+      const int64_t xg = x0 - a.t1.anchor + c;
+      const bool xout = (xg < 0 || xg >= a.Wd) && a.mode1 == HC_MODE_CONSTANT;
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        U[(16 * ch + j) * kSepCols + c] = xout ? a.cval : roundtrip<W>(acc[j], a.out_dt);
+    }
+    __syncthreads();
+    // pass 2 (axis 1) along the rows of the intermediate
+    for (int it = threadIdx.x; it < TY * nq; it += 256) {
+      const int i = it / nq, o = (it - i * nq) * 4;
+      const int64_t y = y0 + i, x = x0 + o;
+      if (y >= a.H || x >= a.Wd) continue;
+      W acc[4] = {(W)0, (W)0, (W)0, (W)0};
+      conv_row4<W>(acc, U + i * kSepCols + o, a.t1);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (x + j < a.Wd) store_cast<W>(a.out, y * a.Wd + x + j, a.out_dt, acc[j]);
+    }
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------- host side --
+namespace {
+
+bool valid_dtype(int dt) { return dt >= HC_BOOL && dt <= HC_FLOAT64; }
+bool valid_mode(int m) { return m >= HC_MODE_REFLECT && m <= HC_MODE_CONSTANT; }
+
+int check_in_out(const hc_array* in, const hc_array* out) {
+  if (!in || !out) return fail(HC_ERR_VALUE, "null array");
+  if (in->ndim < 1 || in->ndim > HC_MAX_NDIM) return fail(HC_ERR_VALUE, "unsupported ndim");
+  if (!valid_dtype(in->dtype) || !valid_dtype(out->dtype))
+    return fail(HC_ERR_UNSUPPORTED, "unsupported dtype");
+  if (!same_shape(in, out)) return fail(HC_ERR_VALUE, "output shape not correct");
+  if (!is_c_contiguous(out)) return fail(HC_ERR_VALUE, "output must be C-contiguous");
+  return HC_OK;
+}
+
+template <typename W>
+int launch_gather_1d(const hc_array* in, const hc_array* out, const hc_taps* taps, int axis,
+                     int mode, double cval, hipStream_t st) {
+  GatherArgs<W> a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = in->data;
+  a.out = out->data;
+  a.in_dt = in->dtype;
+  a.out_dt = out->dtype;
+  a.ndim = in->ndim;
+  a.total = numel(in);
+  for (int d = 0; d < in->ndim; ++d) {
+    a.shape[d] = in->shape[d];
+    a.istride[d] = in->strides[d];
+    a.wsh[d] = 1;
+    a.off[d] = 0;
+    a.mode[d] = mode;
+  }
+  a.wsh[axis] = (int32_t)taps->K;
+  a.off[axis] = taps->anchor;
+  a.ntaps = taps->K;
+  a.cval = (W)cval;
+  if (taps->K <= inline_cap<W>()) {
+    a.tp.K = (int)taps->K;
+    for (int64_t k = 0; k < taps->K; ++k) a.tp.w[k] = (W)taps->w[k];
+    a.wdev = nullptr;
+  } else {
+    if (!taps->w_dev) return fail(HC_ERR_VALUE, "long kernel needs a device copy of the weights");
+    a.wdev = (const W*)taps->w_dev;
+  }
+  if (a.total == 0) return HC_OK;
+  const int64_t blocks = std::min<int64_t>((a.total + 255) / 256, 1 << 20);
+  HC_LAUNCH("k_gather", st, k_gather<W>, dim3((unsigned)blocks), dim3(256), 0, a);
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+template <typename W>
+int launch_tiled_1d(const hc_array* in, const hc_array* out, const hc_taps* taps, int axis,
+                    int mode, double cval, hipStream_t st) {
+  TiledArgs<W> a;
+  std::memset(&a, 0, sizeof(a));
+  int rc = make_taps<W>(taps, 4, &a.tp);
+  if (rc != HC_OK) return rc;
+  const Lines ln = lines_of(in, axis);
+  a.in = in->data;
+  a.out = out->data;
+  a.in_dt = in->dtype;
+  a.out_dt = out->dtype;
+  a.N = ln.N;
+  a.inner = ln.inner;
+  a.mode = mode;
+  a.cval = (W)cval;
+  if (numel(in) == 0) return HC_OK;
+  if (ln.inner == 1) {
+    a.nlines = ln.outer;
+    a.nseg = (ln.N + kSeg - 1) / kSeg;
+    const int64_t nsegs = a.nlines * a.nseg;
+    const int64_t blocks = (nsegs + 3) / 4;
+    if (blocks > 0x7fffffff) return fail(HC_ERR_VALUE, "array too large");
+    const size_t smem = (size_t)4 * (kSeg + a.tp.K + 4) * sizeof(W);
+    HC_LAUNCH("k_tiled_contig", st, k_tiled_contig<W>, dim3((unsigned)blocks), dim3(256), smem, a);
+  } else {
+    a.nlines = ln.outer * ln.inner;
+    a.nseg = (ln.N + kStrTA - 1) / kStrTA;
+    const int64_t nlb = (a.nlines + kStrCols - 1) / kStrCols;
+    const int64_t blocks = nlb * a.nseg;
+    if (blocks > 0x7fffffff) return fail(HC_ERR_VALUE, "array too large");
+    const size_t smem = (size_t)(kStrTA + a.tp.K) * kStrStride * sizeof(W);
+    HC_LAUNCH("k_tiled_strided", st, k_tiled_strided<W>, dim3((unsigned)blocks), dim3(256), smem, a);
+  }
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+template <typename W>
+int launch_gather_nd(const hc_array* in, const hc_array* out, const void* w_dev,
+                     const int64_t* wshape, const int32_t* offsets, const int32_t* modes,
+                     double cval, hipStream_t st) {
+  GatherArgs<W> a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = in->data;
+  a.out = out->data;
+  a.in_dt = in->dtype;
+  a.out_dt = out->dtype;
+  a.ndim = in->ndim;
+  a.total = numel(in);
+  a.ntaps = 1;
+  a.all_const = 1;
+  for (int d = 0; d < in->ndim; ++d) {
+    a.shape[d] = in->shape[d];
+    a.istride[d] = in->strides[d];
+    if (wshape[d] < 1) return fail(HC_ERR_VALUE, "filter weights array has incorrect shape");
+    a.wsh[d] = (int32_t)wshape[d];
+    a.off[d] = offsets[d];
+    if (offsets[d] < 0 || offsets[d] >= wshape[d]) return fail(HC_ERR_VALUE, "invalid origin");
+    if (!valid_mode(modes[d])) return fail(HC_ERR_VALUE, "boundary mode not supported");
+    a.mode[d] = modes[d];
+    a.ntaps *= wshape[d];
+  }
+  if (!w_dev) return fail(HC_ERR_VALUE, "weights device pointer is null");
+  a.wdev = (const W*)w_dev;
+  a.cval = (W)cval;
+  if (a.total == 0) return HC_OK;
+  const int64_t blocks = std::min<int64_t>((a.total + 255) / 256, 1 << 20);
+  HC_LAUNCH("k_gather", st, k_gather<W>, dim3((unsigned)blocks), dim3(256), 0, a);
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+template <typename W, int TY>
+int launch_sep2d_ty(const Sep2dArgs<W>& a, hipStream_t st, size_t smem) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void*)k_sep2d<W, TY>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr_set = true;
+  }
+  const int64_t grid = std::min<int64_t>(a.ntiles, 256 * 8);
+  HC_LAUNCH("k_sep2d", st, (k_sep2d<W, TY>), dim3((unsigned)grid), dim3(256), smem, a);
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+template <typename W>
+int launch_sep2d(const hc_array* in, const hc_array* out, const hc_taps* t0, const hc_taps* t1,
+                 int mode0, int mode1, double cval, hipStream_t st) {
+  Sep2dArgs<W> a;
+  std::memset(&a, 0, sizeof(a));
+  int rc = make_taps<W>(t0, 4, &a.t0);
+  if (rc != HC_OK) return rc;
+  rc = make_taps<W>(t1, 4, &a.t1);
+  if (rc != HC_OK) return rc;
+  if (a.t1.K > kSepCols - 20 || a.t0.K > 128)
+    return fail(HC_ERR_RESOURCE, "kernels too long for the fused 2-D tile");
+  a.in = in->data;
+  a.out = out->data;
+  a.in_dt = in->dtype;
+  a.out_dt = out->dtype;
+  a.H = in->shape[0];
+  a.Wd = in->shape[1];
+  a.TX = kSepCols - a.t1.K - 4;  // row reads run 8 past the last group
+  a.mode0 = mode0;
+  a.mode1 = mode1;
+  a.cval = (W)cval;
+  if (a.H == 0 || a.Wd == 0) return HC_OK;
+  constexpr int TY = sizeof(W) == 4 ? 32 : 16;
+  a.ntx = (a.Wd + a.TX - 1) / a.TX;
+  const int64_t nty = (a.H + TY - 1) / TY;
+  a.ntiles = a.ntx * nty;
+  const size_t smem = (size_t)(TY + a.t0.K + TY) * kSepCols * sizeof(W);
+  return launch_sep2d_ty<W, TY>(a, st, smem);
+}
+
+}  // namespace
+}  // namespace hc
+
+using namespace hc;
+
+extern "C" {
+
+int hc_correlate1d(const hc_array* in, const hc_array* out, const hc_taps* taps, int axis,
+                   int mode, double cval, int algorithm, void* stream) {
+  int rc = check_in_out(in, out);
+  if (rc != HC_OK) return rc;
+  if (!taps) return fail(HC_ERR_VALUE, "null taps");
+  if (axis < 0) axis += in->ndim;
+  if (axis < 0 || axis >= in->ndim) return fail(HC_ERR_VALUE, "invalid axis");
+  if (!valid_mode(mode)) return fail(HC_ERR_VALUE, "boundary mode not supported");
+  if (taps->K < 1) return fail(HC_ERR_VALUE, "incorrect filter size");
+  if (taps->anchor < 0 || taps->anchor >= taps->K) return fail(HC_ERR_VALUE, "invalid origin");
+  hipStream_t st = (hipStream_t)stream;
+  const bool f64 = taps->w_f64 != 0;
+  const bool contig = is_c_contiguous(in);
+  const int64_t Kp = (taps->K + 3) / 4 * 4;
+  const bool fits = Kp <= (f64 ? inline_cap<double>() : inline_cap<float>());
+  if (algorithm == HC_ALGO_TILED) {
+    if (!contig) return fail(HC_ERR_VALUE, "tiled kernels need a C-contiguous input");
+    if (!fits) return fail(HC_ERR_RESOURCE, "kernel too long for the LDS-tiled line kernels");
+  }
+  const bool tiled = algorithm != HC_ALGO_GATHER && contig && fits;
+  if (tiled)
+    return f64 ? launch_tiled_1d<double>(in, out, taps, axis, mode, cval, st)
+               : launch_tiled_1d<float>(in, out, taps, axis, mode, cval, st);
+  return f64 ? launch_gather_1d<double>(in, out, taps, axis, mode, cval, st)
+             : launch_gather_1d<float>(in, out, taps, axis, mode, cval, st);
+}
+
+int hc_correlate_nd(const hc_array* in, const hc_array* out, const void* w_dev,
+                    const int64_t* wshape, const int32_t* offsets, const int32_t* modes, int w_f64,
+                    double cval, void* stream) {
+  int rc = check_in_out(in, out);
+  if (rc != HC_OK) return rc;
+  if (!wshape || !offsets || !modes) return fail(HC_ERR_VALUE, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  return w_f64 ? launch_gather_nd<double>(in, out, w_dev, wshape, offsets, modes, cval, st)
+               : launch_gather_nd<float>(in, out, w_dev, wshape, offsets, modes, cval, st);
+}
+
+int hc_separable2d(const hc_array* in, const hc_array* out, const hc_taps* taps0,
+                   const hc_taps* taps1, int mode0, int mode1, double cval, void* stream) {
+  int rc = check_in_out(in, out);
+  if (rc != HC_OK) return rc;
+  if (in->ndim != 2) return fail(HC_ERR_VALUE, "hc_separable2d needs a 2-D array");
+  if (!is_c_contiguous(in)) return fail(HC_ERR_VALUE, "input must be C-contiguous");
+  if (!valid_mode(mode0) || !valid_mode(mode1))
+    return fail(HC_ERR_VALUE, "boundary mode not supported");
+  if (!taps0 || !taps1) return fail(HC_ERR_VALUE, "null taps");
+  if (taps0->w_f64 != taps1->w_f64) return fail(HC_ERR_VALUE, "weights dtypes differ");
+  hipStream_t st = (hipStream_t)stream;
+  return taps0->w_f64 ? launch_sep2d<double>(in, out, taps0, taps1, mode0, mode1, cval, st)
+                      : launch_sep2d<float>(in, out, taps0, taps1, mode0, mode1, cval, st);
+}
+
+}  // extern "C"

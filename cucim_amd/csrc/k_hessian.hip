@@ -1,0 +1,967 @@
+// Fused Hessian-by-Gaussian-derivatives, eigenvalues and Frangi vesselness.
+//
+// Reference (paths relative to python/cucim/src/cucim/skimage/):
+//   feature/corner.py:161-254  _hessian_matrix_with_gaussian: gradients
+//       g_a = gaussian_filter(I, order=e_a) then H_ab = gaussian_filter(g_a,
+//       order=e_b); every 1-D pass re-extends its input at the border.
+//   feature/corner.py:473-756  closed-form eigenvalues
+//   filters/ridges.py:322-476  frangi (gamma from the first sigma, running max)
+//
+// Restructuring.  Along one axis x the reference applies two 1-D passes to
+// each H_ab: first d (order-1 kernel) if x == a else s (order-0), then d if
+// x == b else s.  Passes along different axes commute exactly, so H_ab is a
+// product over axes of per-axis "chains" (first op, second op).  One launch
+// per axis keeps whole lines resident in LDS, computes the first ops, re-
+// extends their results with the boundary mode (exactly what the
+// reference's second pass sees), applies the second ops and writes only the
+// distinct chain results.  3-D: axis 0 (3 chains from the image), axis 1
+// (6 chains), axis 2 (the six H elements, consumed in registers by the
+// eigenvalue / Frangi epilogue).  Intermediates live in the caller's H (or
+// work) arrays, in place, because each work-group reads its lines completely
+// before writing the same lines.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "hc_conv.h"
+#include "hc_eig.h"
+
+namespace hc {
+
+constexpr int kChC = 16;        // strided chain pass: lines per work-group
+constexpr int kChS = kChC + 1;  // LDS row stride
+constexpr int kFinIPT = 4;      // final pass: 4-output items per thread
+constexpr int kMaxLds = 160 * 1024;
+
+struct ChainOut {
+  void* out;
+  int first;   // 0: smoothing (order 0), 1: derivative (order 1)
+  int second;  // kernel of the second pass
+};
+
+struct ChainJob {
+  const void* in;
+  int in_dt;
+  int nout;
+  ChainOut o[3];
+};
+
+template <typename W>
+struct ChainArgs {
+  ChainJob job[3];
+  int64_t N, inner, nlines;
+  int E;        // LDS line length: roundup(N) + K
+  int Nr;       // N rounded up to the output block
+  int rows;     // contiguous variant: lines per work-group
+  int mode;
+  int negate;   // first pass of Frangi with black_ridges=False: image = -image
+  W cval;
+  TapPack<W> S, D;  // same padded length and anchor
+};
+
+// --------------------------------------------------- strided chain pass --
+template <typename W>
+__global__ __launch_bounds__(256) void k_chain_strided(const ChainArgs<W> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  W* xin = reinterpret_cast<W*>(smem_raw);
+  const int E = a.E, Nr = a.Nr;
+  W* xs = xin + E * kChS;
+  W* xd = xs + E * kChS;
+  const ChainJob& jb = a.job[blockIdx.y];
+  const int anc = a.S.anchor;
+  const int64_t L0 = (int64_t)blockIdx.x * kChC;
+  const int tid = threadIdx.x;
+  bool need_s = false, need_d = false;
+  for (int k = 0; k < jb.nout; ++k) {
+    need_s |= jb.o[k].first == 0;
+    need_d |= jb.o[k].first == 1;
+  }
+  // 1. stage whole lines (+ halo) of the job's input
+  for (int e = tid; e < E * kChC; e += 256) {
+    const int q = e / kChC, c = e - q * kChC;
+    const int64_t L = L0 + c;
+    W v = a.cval;
+    if (L < a.nlines) {
+      const int64_t r = remap((int64_t)q - anc, a.N, a.mode);
+      if (r >= 0) {
+        const int64_t lo = L / a.inner, li = L - lo * a.inner;
+        v = load_as<W>(jb.in, lo * a.N * a.inner + li + r * a.inner, jb.in_dt);
+        if (a.negate) v = -v;
+      }
+    }
+    xin[q * kChS + c] = v;
+  }
+  __syncthreads();
+  // 2. first passes over positions [0, Nr)
+  const int nitems = (Nr / 16) * kChC;
+  for (int it = tid; it < nitems; it += 256) {
+    const int ch = it / kChC, c = it - ch * kChC;
+    const W* x = xin + (16 * ch) * kChS + c;
+    W as[16], ad[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) { as[j] = (W)0; ad[j] = (W)0; }
+    if (need_s && need_d) conv_col16x2<W, kChS>(as, ad, x, a.S, a.D);
+    else if (need_s) conv_col16<W, kChS>(as, x, a.S);
+    else conv_col16<W, kChS>(ad, x, a.D);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int q = anc + 16 * ch + j;
+      xs[q * kChS + c] = as[j];
+      xd[q * kChS + c] = ad[j];
+    }
+  }
+  __syncthreads();
+  // 3. boundary-extend the first-pass results
+  const int nh = E - a.N;  // positions [0, anc) and [anc + N, E)
+  for (int e = tid; e < nh * kChC; e += 256) {
+    const int h = e / kChC, c = e - h * kChC;
+    const int q = h < anc ? h : h + (int)a.N;
+    const int64_t r = remap((int64_t)q - anc, a.N, a.mode);
+    W vs = a.cval, vd = a.cval;
+    if (r >= 0) {
+      vs = xs[(r + anc) * kChS + c];
+      vd = xd[(r + anc) * kChS + c];
+    }
+    xs[q * kChS + c] = vs;
+    xd[q * kChS + c] = vd;
+  }
+  __syncthreads();
+  // 4. second passes, sharing window reads between outputs of one source
+  for (int it = tid; it < nitems; it += 256) {
+    const int ch = it / kChC, c = it - ch * kChC;
+    const int64_t L = L0 + c;
+    if (L >= a.nlines) continue;
+    const int64_t lo = L / a.inner, li = L - lo * a.inner;
+    const int64_t base = lo * a.N * a.inner + li;
+#pragma unroll 1
+    for (int src = 0; src < 2; ++src) {
+      int ks = -1, kd = -1;
+      for (int k = 0; k < jb.nout; ++k) {
+        if (jb.o[k].first != src) continue;
+        if (jb.o[k].second == 0) ks = k; else kd = k;
+      }
+      if (ks < 0 && kd < 0) continue;
+      const W* x = (src ? xd : xs) + (16 * ch) * kChS + c;
+      W r0[16], r1[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) { r0[j] = (W)0; r1[j] = (W)0; }
+      if (ks >= 0 && kd >= 0) conv_col16x2<W, kChS>(r0, r1, x, a.S, a.D);
+      else if (ks >= 0) conv_col16<W, kChS>(r0, x, a.S);
+      else conv_col16<W, kChS>(r1, x, a.D);
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int64_t n = 16 * ch + j;
+        if (n < a.N) {
+          if (ks >= 0) reinterpret_cast<W*>(jb.o[ks].out)[base + n * a.inner] = r0[j];
+          if (kd >= 0) reinterpret_cast<W*>(jb.o[kd].out)[base + n * a.inner] = r1[j];
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------ contiguous chain pass --
+// Only used when the first Hessian pass runs along a contiguous axis (1-D
+// images are not on this path; kept for 2-D arrays whose axis-0 lines are
+// too long for the strided pass).  Lines are rows.
+template <typename W>
+__global__ __launch_bounds__(256) void k_chain_contig(const ChainArgs<W> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int E = a.E, Nr = a.Nr, R = a.rows;
+  W* xin = reinterpret_cast<W*>(smem_raw);
+  W* xs = xin + R * E;
+  W* xd = xs + R * E;
+  const ChainJob& jb = a.job[blockIdx.y];
+  const int anc = a.S.anchor;
+  const int64_t row0 = (int64_t)blockIdx.x * R;
+  const int tid = threadIdx.x;
+  bool need_s = false, need_d = false;
+  for (int k = 0; k < jb.nout; ++k) {
+    need_s |= jb.o[k].first == 0;
+    need_d |= jb.o[k].first == 1;
+  }
+  for (int e = tid; e < R * E; e += 256) {
+    const int rr = e / E, q = e - rr * E;
+    const int64_t row = row0 + rr;
+    W v = a.cval;
+    if (row < a.nlines) {
+      const int64_t r = remap((int64_t)q - anc, a.N, a.mode);
+      if (r >= 0) {
+        v = load_as<W>(jb.in, row * a.N + r, jb.in_dt);
+        if (a.negate) v = -v;
+      }
+    }
+    xin[e] = v;
+  }
+  __syncthreads();
+  const int nq = Nr / 4;
+  for (int it = tid; it < R * nq; it += 256) {
+    const int rr = it / nq, o = (it - rr * nq) * 4;
+    W as[4] = {(W)0, (W)0, (W)0, (W)0}, ad[4] = {(W)0, (W)0, (W)0, (W)0};
+    const W* x = xin + rr * E + o;
+    if (need_s && need_d) conv_row4x2<W>(as, ad, x, a.S, a.D);
+    else if (need_s) conv_row4<W>(as, x, a.S);
+    else conv_row4<W>(ad, x, a.D);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      xs[rr * E + anc + o + j] = as[j];
+      xd[rr * E + anc + o + j] = ad[j];
+    }
+  }
+  __syncthreads();
+  const int nh = E - (int)a.N;
+  for (int e = tid; e < R * nh; e += 256) {
+    const int rr = e / nh, h = e - rr * nh;
+    const int q = h < anc ? h : h + (int)a.N;
+    const int64_t r = remap((int64_t)q - anc, a.N, a.mode);
+    W vs = a.cval, vd = a.cval;
+    if (r >= 0) {
+      vs = xs[rr * E + r + anc];
+      vd = xd[rr * E + r + anc];
+    }
+    xs[rr * E + q] = vs;
+    xd[rr * E + q] = vd;
+  }
+  __syncthreads();
+  for (int it = tid; it < R * nq; it += 256) {
+    const int rr = it / nq, o = (it - rr * nq) * 4;
+    const int64_t row = row0 + rr;
+    if (row >= a.nlines) continue;
+#pragma unroll 1
+    for (int src = 0; src < 2; ++src) {
+      int ks = -1, kd = -1;
+      for (int k = 0; k < jb.nout; ++k) {
+        if (jb.o[k].first != src) continue;
+        if (jb.o[k].second == 0) ks = k; else kd = k;
+      }
+      if (ks < 0 && kd < 0) continue;
+      const W* x = (src ? xd : xs) + rr * E + o;
+      W r0[4] = {(W)0, (W)0, (W)0, (W)0}, r1[4] = {(W)0, (W)0, (W)0, (W)0};
+      if (ks >= 0 && kd >= 0) conv_row4x2<W>(r0, r1, x, a.S, a.D);
+      else if (ks >= 0) conv_row4<W>(r0, x, a.S);
+      else conv_row4<W>(r1, x, a.D);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (o + j < a.N) {
+          if (ks >= 0) reinterpret_cast<W*>(jb.o[ks].out)[row * a.N + o + j] = r0[j];
+          if (kd >= 0) reinterpret_cast<W*>(jb.o[kd].out)[row * a.N + o + j] = r1[j];
+        }
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------- final pass --
+enum Epilogue { EPI_STORE = 0, EPI_FRANGI = 1 };
+
+template <typename W>
+struct FinalArgs {
+  const W* in[6];
+  int first[6], second[6];
+  int nin;          // 3 (2-D) or 6 (3-D)
+  int64_t N, nrows;
+  int E, Nr, rows;
+  int mode;
+  W cval;
+  TapPack<W> S, D;
+  int epi;
+  W* out[6];        // EPI_STORE targets (may alias in[])
+  // EPI_FRANGI
+  W* fout;          // vesselness running maximum
+  W* eig[3];        // eigenvalue store (first sigma, gamma unknown)
+  int store_eig;    // 1: write eig + reduce max r_g; 0: vesselness now
+  int first_sigma;  // 1: fout = v, else fout = max(fout, v)
+  double alpha_sq, beta_sq, gamma_sq;
+  const double* gamma_sq_dev;  // when non-null, overrides gamma_sq
+  void* rg_max;     // uint32 (float) / uint64 (double) bit pattern
+};
+
+template <typename W>
+__device__ __forceinline__ void atomic_max_bits(void* p, W v);
+template <>
+__device__ __forceinline__ void atomic_max_bits<float>(void* p, float v) {
+  unsigned int b = __float_as_uint(v);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const unsigned int x = __shfl_xor(b, o, 64);
+    b = x > b ? x : b;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(reinterpret_cast<unsigned int*>(p), b);
+}
+template <>
+__device__ __forceinline__ void atomic_max_bits<double>(void* p, double v) {
+  unsigned long long b = (unsigned long long)__double_as_longlong(v);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const unsigned long long x = __shfl_xor(b, o, 64);
+    b = x > b ? x : b;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(reinterpret_cast<unsigned long long*>(p), b);
+}
+
+template <typename W, int NIN>
+__global__ __launch_bounds__(256) void k_final_contig(const FinalArgs<W> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int E = a.E, Nr = a.Nr, R = a.rows;
+  W* xin = reinterpret_cast<W*>(smem_raw);
+  W* xf = xin + R * E;
+  const int anc = a.S.anchor;
+  const int64_t row0 = (int64_t)blockIdx.x * R;
+  const int tid = threadIdx.x;
+  const int nq = Nr / 4;
+  const int nitems = R * nq;
+  W res[NIN][kFinIPT][4];
+#pragma unroll
+  for (int j = 0; j < NIN; ++j) {
+    for (int e = tid; e < R * E; e += 256) {
+      const int rr = e / E, q = e - rr * E;
+      const int64_t row = row0 + rr;
+      W v = a.cval;
+      if (row < a.nrows) {
+        const int64_t r = remap((int64_t)q - anc, a.N, a.mode);
+        if (r >= 0) v = a.in[j][row * a.N + r];
+      }
+      xin[e] = v;
+    }
+    __syncthreads();
+    const TapPack<W>& t1 = a.first[j] ? a.D : a.S;
+#pragma unroll
+    for (int i = 0; i < kFinIPT; ++i) {
+      const int it = tid + 256 * i;
+      if (it < nitems) {
+        const int rr = it / nq, o = (it - rr * nq) * 4;
+        W acc[4] = {(W)0, (W)0, (W)0, (W)0};
+        conv_row4<W>(acc, xin + rr * E + o, t1);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) xf[rr * E + anc + o + m] = acc[m];
+      }
+    }
+    __syncthreads();
+    const int nh = E - (int)a.N;
+    for (int e = tid; e < R * nh; e += 256) {
+      const int rr = e / nh, h = e - rr * nh;
+      const int q = h < anc ? h : h + (int)a.N;
+      const int64_t r = remap((int64_t)q - anc, a.N, a.mode);
+      xf[rr * E + q] = r >= 0 ? xf[rr * E + r + anc] : a.cval;
+    }
+    __syncthreads();
+    const TapPack<W>& t2 = a.second[j] ? a.D : a.S;
+#pragma unroll
+    for (int i = 0; i < kFinIPT; ++i) {
+      const int it = tid + 256 * i;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) res[j][i][m] = (W)0;
+      if (it < nitems) {
+        const int rr = it / nq, o = (it - rr * nq) * 4;
+        conv_row4<W>(res[j][i], xf + rr * E + o, t2);
+      }
+    }
+    __syncthreads();
+  }
+  // epilogue: every item holds all Hessian elements of its 4 voxels
+  double gsq = a.gamma_sq;
+  if (a.epi == EPI_FRANGI && a.gamma_sq_dev) gsq = *a.gamma_sq_dev;
+  W rgmax = (W)0;
+#pragma unroll
+  for (int i = 0; i < kFinIPT; ++i) {
+    const int it = tid + 256 * i;
+    if (it >= nitems) continue;
+    const int rr = it / nq, o = (it - rr * nq) * 4;
+    const int64_t row = row0 + rr;
+    if (row >= a.nrows) continue;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      if (o + m >= a.N) continue;
+      const int64_t idx = row * a.N + o + m;
+      if (a.epi == EPI_STORE) {
+#pragma unroll
+        for (int j = 0; j < NIN; ++j) a.out[j][idx] = res[j][i][m];
+        continue;
+      }
+      if (NIN == 6) {
+        W l1, l2, l3;
+        // H order (H00, H01, H02, H11, H12, H22) = (a, d, f, b, e, c)
+        eig33<W>(res[0][i][m], res[3][i][m], res[5][i][m], res[1][i][m], res[4][i][m],
+                 res[2][i][m], 0, 1, l1, l2, l3);
+        const W rg = frangi_rg3<W>(l1, l2, l3);
+        if (a.store_eig) {
+          a.eig[0][idx] = l1;
+          a.eig[1][idx] = l2;
+          a.eig[2][idx] = l3;
+          rgmax = nanmax<W>(rgmax, rg);
+        } else {
+          const W v = frangi_v3<W>(l1, l2, l3, rg, a.alpha_sq, a.beta_sq, gsq);
+          a.fout[idx] = a.first_sigma ? v : nanmax<W>(a.fout[idx], v);
+        }
+      } else {
+        W l1, l2;
+        eig22<W>(res[0][i][m], res[1][i][m], res[2][i][m], 0, 1, l1, l2);
+        const W rg = frangi_rg2<W>(l1, l2);
+        if (a.store_eig) {
+          a.eig[0][idx] = l1;
+          a.eig[1][idx] = l2;
+          rgmax = nanmax<W>(rgmax, rg);
+        } else {
+          const W v = frangi_v2<W>(l1, l2, rg, a.beta_sq, gsq);
+          a.fout[idx] = a.first_sigma ? v : nanmax<W>(a.fout[idx], v);
+        }
+      }
+    }
+  }
+  if (a.epi == EPI_FRANGI && a.store_eig) atomic_max_bits<W>(a.rg_max, rgmax);
+}
+
+// ---------------------------------------------------- small kernels --
+__global__ void k_frangi_reset(unsigned long long* rg) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) rg[0] = 0ull;
+}
+
+template <typename W>
+__global__ void k_frangi_gamma(const unsigned long long* rg, double* gamma_sq) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s;
+  if (sizeof(W) == 4) s = (double)__uint_as_float((unsigned int)(rg[0] & 0xffffffffull));
+  else s = __longlong_as_double((long long)rg[0]);
+  gamma_sq[0] = gamma_sq_from_max(s);
+}
+
+template <typename W, int NDIM>
+__global__ __launch_bounds__(256) void k_frangi_from_eig(const W* __restrict__ e0,
+                                                         const W* __restrict__ e1,
+                                                         const W* __restrict__ e2,
+                                                         W* __restrict__ out, int64_t n,
+                                                         double alpha_sq, double beta_sq,
+                                                         const double* gamma_sq_dev) {
+  const double gsq = *gamma_sq_dev;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    if (NDIM == 3) {
+      const W l1 = e0[i], l2 = e1[i], l3 = e2[i];
+      out[i] = frangi_v3<W>(l1, l2, l3, frangi_rg3<W>(l1, l2, l3), alpha_sq, beta_sq, gsq);
+    } else {
+      const W l1 = e0[i], l2 = e1[i];
+      out[i] = frangi_v2<W>(l1, l2, frangi_rg2<W>(l1, l2), beta_sq, gsq);
+    }
+  }
+}
+
+// vesselness from stored eigenvalues, with the running maximum
+template <typename W, int NDIM>
+__global__ __launch_bounds__(256) void k_frangi_eig_max(const W* __restrict__ e0,
+                                                        const W* __restrict__ e1,
+                                                        const W* __restrict__ e2,
+                                                        W* __restrict__ out, int64_t n,
+                                                        double alpha_sq, double beta_sq,
+                                                        double gamma_sq,
+                                                        const double* gamma_sq_dev, int first) {
+  const double gsq = gamma_sq_dev ? *gamma_sq_dev : gamma_sq;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    W v;
+    if (NDIM == 3) {
+      const W l1 = e0[i], l2 = e1[i], l3 = e2[i];
+      v = frangi_v3<W>(l1, l2, l3, frangi_rg3<W>(l1, l2, l3), alpha_sq, beta_sq, gsq);
+    } else {
+      const W l1 = e0[i], l2 = e1[i];
+      v = frangi_v2<W>(l1, l2, frangi_rg2<W>(l1, l2), beta_sq, gsq);
+    }
+    out[i] = first ? v : nanmax<W>(out[i], v);
+  }
+}
+
+// max over voxels of r_g = sum lambda^2 (bit-pattern max, NaN wins)
+template <typename W, int NDIM>
+__global__ __launch_bounds__(256) void k_rg_max(const W* __restrict__ e0, const W* __restrict__ e1,
+                                                const W* __restrict__ e2, int64_t n, void* rg) {
+  W m = (W)0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    const W r = NDIM == 3 ? frangi_rg3<W>(e0[i], e1[i], e2[i]) : frangi_rg2<W>(e0[i], e1[i]);
+    m = nanmax<W>(m, r);
+  }
+  atomic_max_bits<W>(rg, m);
+}
+
+template <typename W>
+__global__ __launch_bounds__(256) void k_sym_eig(const W* __restrict__ s0, const W* __restrict__ s1,
+                                                 const W* __restrict__ s2, const W* __restrict__ s3,
+                                                 const W* __restrict__ s4, const W* __restrict__ s5,
+                                                 W* __restrict__ e0, W* __restrict__ e1,
+                                                 W* __restrict__ e2, int64_t n, int nS,
+                                                 int descending, int abs_sort) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    if (nS == 6) {
+      W l1, l2, l3;
+      // (S0..S5) = (M00, M01, M02, M11, M12, M22) -> (a, d, f, b, e, c)
+      eig33<W>(s0[i], s3[i], s5[i], s1[i], s4[i], s2[i], descending, abs_sort, l1, l2, l3);
+      e0[i] = l1;
+      e1[i] = l2;
+      e2[i] = l3;
+    } else {
+      W l1, l2;
+      eig22<W>(s0[i], s1[i], s2[i], descending, abs_sort, l1, l2);
+      e0[i] = l1;
+      e1[i] = l2;
+    }
+  }
+}
+
+// ------------------------------------------------------------ planning --
+namespace {
+
+struct Pair {
+  int a, b;  // first derivative axis, second derivative axis
+};
+
+// outputs in combinations_with_replacement order of the (possibly reversed)
+// axes (corner.py:246-253)
+int hessian_pairs(int ndim, int order_xy, Pair* p) {
+  int ax[3];
+  for (int d = 0; d < ndim; ++d) ax[d] = order_xy ? ndim - 1 - d : d;
+  int n = 0;
+  for (int i = 0; i < ndim; ++i)
+    for (int j = i; j < ndim; ++j) p[n++] = Pair{ax[i], ax[j]};
+  return n;
+}
+
+inline int chain_first(const Pair& p, int x) { return p.a == x ? 1 : 0; }
+inline int chain_second(const Pair& p, int x) { return p.b == x ? 1 : 0; }
+
+template <typename W>
+int lds_strided(int E) { return 3 * E * kChS * (int)sizeof(W); }
+
+template <typename W>
+int pick_rows(int E, int Nr, int nbuf, int max_items) {
+  // lines per work-group: as many as fit max_items 4-output items and ~64 KiB
+  const int nq = Nr / 4;
+  int r = std::max(1, max_items / nq);
+  while (r > 1 && nbuf * r * E * (int)sizeof(W) > 64 * 1024) --r;
+  return r;
+}
+
+template <typename W>
+struct AxisTaps {
+  TapPack<W> S, D;
+};
+
+template <typename W>
+int axis_taps(const hc_taps* ts, const hc_taps* td, AxisTaps<W>* out) {
+  if (ts->K != td->K || ts->anchor != td->anchor)
+    return fail(HC_ERR_VALUE, "order-0 and order-1 kernels must have equal length and anchor");
+  int rc = make_taps<W>(ts, 4, &out->S);
+  if (rc != HC_OK) return rc;
+  rc = make_taps<W>(td, 4, &out->D);
+  if (rc != HC_OK) return rc;
+  if (out->S.K > 128) return fail(HC_ERR_RESOURCE, "kernel too long for the fused Hessian");
+  return HC_OK;
+}
+
+template <typename W>
+int run_chain_pass(const ChainJob* jobs, int njobs, const hc_array* shape_src, int axis,
+                   const AxisTaps<W>& tp, int mode, double cval, int negate, hipStream_t st) {
+  ChainArgs<W> a;
+  std::memset(&a, 0, sizeof(a));
+  for (int j = 0; j < njobs; ++j) a.job[j] = jobs[j];
+  const Lines ln = lines_of(shape_src, axis);
+  a.N = ln.N;
+  a.inner = ln.inner;
+  a.mode = mode;
+  a.negate = negate;
+  a.cval = (W)cval;
+  a.S = tp.S;
+  a.D = tp.D;
+  if (ln.inner > 1) {
+    a.nlines = ln.outer * ln.inner;
+    a.Nr = (int)((ln.N + 15) / 16 * 16);
+    a.E = a.Nr + a.S.K;
+    const int lds = lds_strided<W>(a.E);
+    if (lds > kMaxLds) return fail(HC_ERR_RESOURCE, "line too long for the fused Hessian");
+    const int64_t blocks = (a.nlines + kChC - 1) / kChC;
+    HC_LAUNCH("k_chain_strided", st, k_chain_strided<W>, dim3((unsigned)blocks, njobs), dim3(256), lds, a);
+  } else {
+    a.nlines = ln.outer;
+    a.Nr = (int)((ln.N + 3) / 4 * 4);
+    a.E = a.Nr + a.S.K + 4;
+    a.rows = pick_rows<W>(a.E, a.Nr, 3, 1024);
+    const int lds = 3 * a.rows * a.E * (int)sizeof(W);
+    if (lds > kMaxLds) return fail(HC_ERR_RESOURCE, "line too long for the fused Hessian");
+    const int64_t blocks = (a.nlines + a.rows - 1) / a.rows;
+    HC_LAUNCH("k_chain_contig", st, k_chain_contig<W>, dim3((unsigned)blocks, njobs), dim3(256), lds, a);
+  }
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+template <typename W>
+int prepare_final(FinalArgs<W>* a, const hc_array* shape_src, const AxisTaps<W>& tp, int mode,
+                  double cval) {
+  const int axis = shape_src->ndim - 1;
+  const Lines ln = lines_of(shape_src, axis);
+  a->N = ln.N;
+  a->nrows = ln.outer;
+  a->Nr = (int)((ln.N + 3) / 4 * 4);
+  a->E = a->Nr + tp.S.K + 4;
+  if (a->Nr / 4 > kFinIPT * 256)
+    return fail(HC_ERR_RESOURCE, "last axis too long for the fused Hessian epilogue");
+  a->rows = pick_rows<W>(a->E, a->Nr, 2, kFinIPT * 256);
+  if (2 * a->rows * a->E * (int)sizeof(W) > kMaxLds)
+    return fail(HC_ERR_RESOURCE, "last axis too long for the fused Hessian epilogue");
+  a->mode = mode;
+  a->cval = (W)cval;
+  a->S = tp.S;
+  a->D = tp.D;
+  return HC_OK;
+}
+
+template <typename W>
+int launch_final(const FinalArgs<W>& a, hipStream_t st) {
+  const int lds = 2 * a.rows * a.E * (int)sizeof(W);
+  const int64_t blocks = (a.nrows + a.rows - 1) / a.rows;
+  if (a.nin == 6)
+    HC_LAUNCH("k_final_contig", st, (k_final_contig<W, 6>), dim3((unsigned)blocks), dim3(256), lds, a);
+  else
+    HC_LAUNCH("k_final_contig", st, (k_final_contig<W, 3>), dim3((unsigned)blocks), dim3(256), lds, a);
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+// Check whether every pass of the fused schedule fits; returns HC_OK or
+// HC_ERR_RESOURCE without launching anything.
+template <typename W>
+int plan_fits(const hc_array* img, const AxisTaps<W>* tp) {
+  const int nd = img->ndim;
+  for (int x = 0; x < nd - 1; ++x) {
+    const int Nr = (int)((img->shape[x] + 15) / 16 * 16);
+    if (lds_strided<W>(Nr + tp[x].S.K) > kMaxLds)
+      return fail(HC_ERR_RESOURCE, "line too long for the fused Hessian");
+  }
+  const int64_t N = img->shape[nd - 1];
+  const int Nr = (int)((N + 3) / 4 * 4);
+  if (Nr / 4 > kFinIPT * 256)
+    return fail(HC_ERR_RESOURCE, "last axis too long for the fused Hessian epilogue");
+  const int E = Nr + tp[nd - 1].S.K + 4;
+  if (2 * E * (int)sizeof(W) > kMaxLds)
+    return fail(HC_ERR_RESOURCE, "last axis too long for the fused Hessian epilogue");
+  return HC_OK;
+}
+
+// Runs the non-final passes of the Hessian schedule from `img` into the
+// buffers buf[k] (one per output pair).  After it, buf[k] holds output k's
+// chains along every axis but the last.
+template <typename W>
+int hessian_prefix(const hc_array* img, W* const* buf, const Pair* P, int nP,
+                   const AxisTaps<W>* tp, int mode, double cval, int negate, hipStream_t st) {
+  const int nd = img->ndim;
+  // pass over axis 0 from the image: distinct chains
+  int node_of[6];
+  int nnode = 0;
+  int node_first[6], node_second[6], node_buf[6];
+  for (int k = 0; k < nP; ++k) {
+    const int f = chain_first(P[k], 0), s = chain_second(P[k], 0);
+    int n = -1;
+    for (int m = 0; m < nnode; ++m)
+      if (node_first[m] == f && node_second[m] == s) n = m;
+    if (n < 0) {
+      n = nnode++;
+      node_first[n] = f;
+      node_second[n] = s;
+      node_buf[n] = k;  // stored in its first child's buffer
+    }
+    node_of[k] = n;
+  }
+  if (nnode > 3) return fail(HC_ERR_VALUE, "internal: too many chains");
+  ChainJob job0;
+  std::memset(&job0, 0, sizeof(job0));
+  job0.in = img->data;
+  job0.in_dt = img->dtype;
+  job0.nout = nnode;
+  for (int m = 0; m < nnode; ++m)
+    job0.o[m] = ChainOut{buf[node_buf[m]], node_first[m], node_second[m]};
+  int rc = run_chain_pass<W>(&job0, 1, img, 0, tp[0], mode, cval, negate, st);
+  if (rc != HC_OK) return rc;
+  if (nd == 3) {
+    // pass over axis 1: one job per axis-0 node
+    ChainJob jobs[3];
+    std::memset(jobs, 0, sizeof(jobs));
+    for (int m = 0; m < nnode; ++m) {
+      jobs[m].in = buf[node_buf[m]];
+      jobs[m].in_dt = sizeof(W) == 4 ? HC_FLOAT32 : HC_FLOAT64;
+      for (int k = 0; k < nP; ++k) {
+        if (node_of[k] != m) continue;
+        if (jobs[m].nout >= 3) return fail(HC_ERR_VALUE, "internal: too many chains");
+        jobs[m].o[jobs[m].nout++] =
+            ChainOut{buf[k], chain_first(P[k], 1), chain_second(P[k], 1)};
+      }
+    }
+    rc = run_chain_pass<W>(jobs, nnode, img, 1, tp[1], mode, cval, 0, st);
+    if (rc != HC_OK) return rc;
+  }
+  return HC_OK;
+}
+
+template <typename W>
+int hessian_impl(const hc_array* in, const hc_array* H, int nH, const hc_taps* ts,
+                 const hc_taps* td, int mode, double cval, int order_xy, hipStream_t st) {
+  const int nd = in->ndim;
+  AxisTaps<W> tp[3];
+  for (int x = 0; x < nd; ++x) {
+    int rc = axis_taps<W>(&ts[x], &td[x], &tp[x]);
+    if (rc != HC_OK) return rc;
+  }
+  int rc = plan_fits<W>(in, tp);
+  if (rc != HC_OK) return rc;
+  Pair P[6];
+  const int nP = hessian_pairs(nd, order_xy, P);
+  if (nP != nH) return fail(HC_ERR_VALUE, "wrong number of Hessian outputs");
+  W* buf[6];
+  for (int k = 0; k < nH; ++k) buf[k] = reinterpret_cast<W*>(H[k].data);
+  if (numel(in) == 0) return HC_OK;
+  rc = hessian_prefix<W>(in, buf, P, nP, tp, mode, cval, 0, st);
+  if (rc != HC_OK) return rc;
+  FinalArgs<W> a;
+  std::memset(&a, 0, sizeof(a));
+  rc = prepare_final<W>(&a, in, tp[nd - 1], mode, cval);
+  if (rc != HC_OK) return rc;
+  a.nin = nH;
+  for (int k = 0; k < nH; ++k) {
+    a.in[k] = buf[k];
+    a.out[k] = buf[k];
+    a.first[k] = chain_first(P[k], nd - 1);
+    a.second[k] = chain_second(P[k], nd - 1);
+  }
+  a.epi = EPI_STORE;
+  return launch_final<W>(a, st);
+}
+
+template <typename W>
+int frangi_impl(const hc_array* in, const hc_array* out, const hc_taps* ts, const hc_taps* td,
+                int nsig, double alpha, double beta, double gamma, int black_ridges, int mode,
+                double cval, const hc_array* work, int nwork, void* scratch, hipStream_t st) {
+  const int nd = in->ndim;
+  const int nH = nd == 3 ? 6 : 3;
+  if (nwork < nH) return fail(HC_ERR_VALUE, "not enough work arrays");
+  for (int k = 0; k < nH; ++k) {
+    if (!same_shape(&work[k], in) || !is_c_contiguous(&work[k]))
+      return fail(HC_ERR_VALUE, "work arrays must be C-contiguous with the image's shape");
+    if (work[k].dtype != out->dtype) return fail(HC_ERR_VALUE, "work dtype mismatch");
+  }
+  std::vector<AxisTaps<W>> tps((size_t)nsig * nd);
+  for (int s = 0; s < nsig; ++s)
+    for (int x = 0; x < nd; ++x) {
+      int rc = axis_taps<W>(&ts[s * nd + x], &td[s * nd + x], &tps[(size_t)s * nd + x]);
+      if (rc != HC_OK) return rc;
+    }
+  for (int s = 0; s < nsig; ++s) {
+    int rc = plan_fits<W>(in, &tps[(size_t)s * nd]);
+    if (rc != HC_OK) return rc;
+  }
+  const int64_t n = numel(in);
+  if (n == 0 || nsig == 0) return HC_OK;
+  if (!scratch) return fail(HC_ERR_VALUE, "scratch is null");
+  unsigned long long* rg = reinterpret_cast<unsigned long long*>(scratch);
+  double* gsq_dev = reinterpret_cast<double*>(reinterpret_cast<char*>(scratch) + 16);
+  const double alpha_sq = 2 * alpha * alpha, beta_sq = 2 * beta * beta;
+  const bool gamma_known = gamma >= 0;
+  const double gamma_sq = gamma_known ? 2 * gamma * gamma : 0.0;
+  Pair P[6];
+  hessian_pairs(nd, 0, P);
+  W* buf[6];
+  for (int k = 0; k < nH; ++k) buf[k] = reinterpret_cast<W*>(work[k].data);
+  W* fo = reinterpret_cast<W*>(out->data);
+  const int negate = black_ridges ? 0 : 1;
+  for (int s = 0; s < nsig; ++s) {
+    const AxisTaps<W>* tp = &tps[(size_t)s * nd];
+    int rc = hessian_prefix<W>(in, buf, P, nH, tp, mode, cval, negate, st);
+    if (rc != HC_OK) return rc;
+    FinalArgs<W> a;
+    std::memset(&a, 0, sizeof(a));
+    rc = prepare_final<W>(&a, in, tp[nd - 1], mode, cval);
+    if (rc != HC_OK) return rc;
+    a.nin = nH;
+    for (int k = 0; k < nH; ++k) {
+      a.in[k] = buf[k];
+      a.first[k] = chain_first(P[k], nd - 1);
+      a.second[k] = chain_second(P[k], nd - 1);
+    }
+    a.epi = EPI_FRANGI;
+    a.fout = fo;
+    a.alpha_sq = alpha_sq;
+    a.beta_sq = beta_sq;
+    a.first_sigma = s == 0;
+    const bool store = (s == 0) && !gamma_known;
+    a.store_eig = store;
+    if (store) {
+      for (int e = 0; e < nd; ++e) a.eig[e] = buf[e];
+      a.rg_max = rg;
+      HC_LAUNCH("k_frangi_reset", st, k_frangi_reset, dim3(1), dim3(64), 0, rg);
+    }
+    if (gamma_known) a.gamma_sq = gamma_sq;
+    else if (s > 0) a.gamma_sq_dev = gsq_dev;
+    rc = launch_final<W>(a, st);
+    if (rc != HC_OK) return rc;
+    if (store) {
+      HC_LAUNCH("k_frangi_gamma", st, k_frangi_gamma<W>, dim3(1), dim3(64), 0, rg, gsq_dev);
+      const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 64);
+      if (nd == 3)
+        HC_LAUNCH("k_frangi_from_eig", st, (k_frangi_from_eig<W, 3>), dim3((unsigned)blocks), dim3(256), 0, buf[0], buf[1], buf[2], fo, n, alpha_sq, beta_sq, gsq_dev);
+      else
+        HC_LAUNCH("k_frangi_from_eig", st, (k_frangi_from_eig<W, 2>), dim3((unsigned)blocks), dim3(256), 0, buf[0], buf[1], buf[1], fo, n, alpha_sq, beta_sq, gsq_dev);
+      HC_HIP(hipGetLastError());
+    }
+  }
+  return HC_OK;
+}
+
+template <typename W>
+int frangi_eig_impl(const hc_array* eig, int nd, const hc_array* out, double alpha, double beta,
+                    double gamma, int first_sigma, void* scratch, hipStream_t st) {
+  const int64_t n = numel(out);
+  if (n == 0) return HC_OK;
+  const double alpha_sq = 2 * alpha * alpha, beta_sq = 2 * beta * beta;
+  const bool known = gamma >= 0;
+  const W* e0 = (const W*)eig[0].data;
+  const W* e1 = (const W*)eig[1].data;
+  const W* e2 = (const W*)eig[nd == 3 ? 2 : 1].data;
+  W* o = (W*)out->data;
+  unsigned long long* rg = reinterpret_cast<unsigned long long*>(scratch);
+  double* gsq_dev = reinterpret_cast<double*>(reinterpret_cast<char*>(scratch) + 16);
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 64);
+  if (!known && first_sigma) {
+    HC_LAUNCH("k_frangi_reset", st, k_frangi_reset, dim3(1), dim3(64), 0, rg);
+    if (nd == 3)
+      HC_LAUNCH("k_rg_max", st, (k_rg_max<W, 3>), dim3((unsigned)blocks), dim3(256), 0, e0, e1, e2, n, (void*)rg);
+    else
+      HC_LAUNCH("k_rg_max", st, (k_rg_max<W, 2>), dim3((unsigned)blocks), dim3(256), 0, e0, e1, e2, n, (void*)rg);
+    HC_LAUNCH("k_frangi_gamma", st, k_frangi_gamma<W>, dim3(1), dim3(64), 0, rg, gsq_dev);
+  }
+  const double gsq = known ? 2 * gamma * gamma : 0.0;
+  const double* gdev = known ? nullptr : gsq_dev;
+  if (nd == 3)
+    HC_LAUNCH("k_frangi_eig_max", st, (k_frangi_eig_max<W, 3>), dim3((unsigned)blocks), dim3(256), 0, e0, e1, e2, o, n, alpha_sq, beta_sq, gsq, gdev, first_sigma);
+  else
+    HC_LAUNCH("k_frangi_eig_max", st, (k_frangi_eig_max<W, 2>), dim3((unsigned)blocks), dim3(256), 0, e0, e1, e2, o, n, alpha_sq, beta_sq, gsq, gdev, first_sigma);
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+int check_float_image(const hc_array* in) {
+  if (!in) return fail(HC_ERR_VALUE, "null array");
+  if (in->ndim != 2 && in->ndim != 3)
+    return fail(HC_ERR_VALUE, "the fused Hessian handles 2-D and 3-D images");
+  if (!is_c_contiguous(in)) return fail(HC_ERR_VALUE, "input must be C-contiguous");
+  for (int d = 0; d < in->ndim; ++d)
+    if (in->shape[d] > (int64_t)1 << 30) return fail(HC_ERR_VALUE, "axis too long");
+  return HC_OK;
+}
+
+}  // namespace
+}  // namespace hc
+
+using namespace hc;
+
+extern "C" {
+
+int hc_max_resident_line(int w_f64) {
+  // strided passes bound it: 3 lines x (N + K) x 17 columns in LDS
+  const int es = w_f64 ? 8 : 4;
+  return kMaxLds / (3 * kChS * es) - 128;
+}
+
+int hc_hessian_gaussian(const hc_array* in, const hc_array* H, int nH, const hc_taps* taps_s,
+                        const hc_taps* taps_d, int mode, double cval, int order_xy,
+                        void* stream) {
+  int rc = check_float_image(in);
+  if (rc != HC_OK) return rc;
+  if (in->dtype != HC_FLOAT32 && in->dtype != HC_FLOAT64)
+    return fail(HC_ERR_VALUE, "hessian input must be float32 or float64");
+  if (order_xy && in->ndim != 2) return fail(HC_ERR_VALUE, "Order 'xy' is only supported for 2D images.");
+  if (mode < HC_MODE_REFLECT || mode > HC_MODE_CONSTANT)
+    return fail(HC_ERR_VALUE, "boundary mode not supported");
+  if (!H || !taps_s || !taps_d) return fail(HC_ERR_VALUE, "null argument");
+  for (int k = 0; k < nH; ++k) {
+    if (!same_shape(&H[k], in) || !is_c_contiguous(&H[k]) || H[k].dtype != in->dtype)
+      return fail(HC_ERR_VALUE, "H arrays must be C-contiguous with the image's shape and dtype");
+  }
+  const bool f64 = in->dtype == HC_FLOAT64;
+  for (int x = 0; x < in->ndim; ++x)
+    if ((taps_s[x].w_f64 != 0) != f64 || (taps_d[x].w_f64 != 0) != f64)
+      return fail(HC_ERR_VALUE, "weights dtype must match the image dtype");
+  hipStream_t st = (hipStream_t)stream;
+  return f64 ? hessian_impl<double>(in, H, nH, taps_s, taps_d, mode, cval, order_xy, st)
+             : hessian_impl<float>(in, H, nH, taps_s, taps_d, mode, cval, order_xy, st);
+}
+
+int hc_sym_eigvals(const hc_array* S, int nS, const hc_array* eig, int sort, int abs_sort,
+                   void* stream) {
+  if (!S || !eig) return fail(HC_ERR_VALUE, "null argument");
+  if (nS != 3 && nS != 6) return fail(HC_ERR_VALUE, "closed-form eigenvalues need 3 or 6 elements");
+  const int ne = nS == 6 ? 3 : 2;
+  const int dt = S[0].dtype;
+  if (dt != HC_FLOAT32 && dt != HC_FLOAT64)
+    return fail(HC_ERR_VALUE, "expected real-valued floating point matrices");
+  for (int k = 0; k < nS; ++k)
+    if (!same_shape(&S[k], &S[0]) || !is_c_contiguous(&S[k]) || S[k].dtype != dt)
+      return fail(HC_ERR_VALUE, "element arrays must be C-contiguous, same shape and dtype");
+  for (int k = 0; k < ne; ++k)
+    if (!same_shape(&eig[k], &S[0]) || !is_c_contiguous(&eig[k]) || eig[k].dtype != dt)
+      return fail(HC_ERR_VALUE, "eigenvalue arrays must match the element arrays");
+  const int64_t n = numel(&S[0]);
+  if (n == 0) return HC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 64);
+  const void* s[6];
+  for (int k = 0; k < 6; ++k) s[k] = S[k < nS ? k : 0].data;
+  void* e[3];
+  for (int k = 0; k < 3; ++k) e[k] = eig[k < ne ? k : 0].data;
+  if (dt == HC_FLOAT64)
+    HC_LAUNCH("k_sym_eig", st, k_sym_eig<double>, dim3((unsigned)blocks), dim3(256), 0, (const double*)s[0], (const double*)s[1], (const double*)s[2], (const double*)s[3], (const double*)s[4], (const double*)s[5], (double*)e[0], (double*)e[1], (double*)e[2], n, nS, sort, abs_sort);
+  else
+    HC_LAUNCH("k_sym_eig", st, k_sym_eig<float>, dim3((unsigned)blocks), dim3(256), 0, (const float*)s[0], (const float*)s[1], (const float*)s[2], (const float*)s[3], (const float*)s[4], (const float*)s[5], (float*)e[0], (float*)e[1], (float*)e[2], n, nS, sort, abs_sort);
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+int hc_frangi_from_eigvals(const hc_array* eig, int ndim, const hc_array* out, double alpha,
+                           double beta, double gamma, int first_sigma, void* scratch,
+                           void* stream) {
+  if (!eig || !out || !scratch) return fail(HC_ERR_VALUE, "null argument");
+  if (ndim != 2 && ndim != 3) return fail(HC_ERR_VALUE, "frangi handles 2-D and 3-D images");
+  if (out->dtype != HC_FLOAT32 && out->dtype != HC_FLOAT64)
+    return fail(HC_ERR_VALUE, "frangi output must be float32 or float64");
+  if (!is_c_contiguous(out)) return fail(HC_ERR_VALUE, "output must be C-contiguous");
+  for (int k = 0; k < ndim; ++k)
+    if (!same_shape(&eig[k], out) || !is_c_contiguous(&eig[k]) || eig[k].dtype != out->dtype)
+      return fail(HC_ERR_VALUE, "eigenvalue arrays must match the output");
+  hipStream_t st = (hipStream_t)stream;
+  return out->dtype == HC_FLOAT64
+             ? frangi_eig_impl<double>(eig, ndim, out, alpha, beta, gamma, first_sigma, scratch, st)
+             : frangi_eig_impl<float>(eig, ndim, out, alpha, beta, gamma, first_sigma, scratch, st);
+}
+
+int hc_frangi(const hc_array* in, const hc_array* out, const hc_taps* taps_s,
+              const hc_taps* taps_d, int nsig, double alpha, double beta, double gamma,
+              int black_ridges, int mode, double cval, const hc_array* work, int nwork,
+              void* scratch, void* stream) {
+  int rc = check_float_image(in);
+  if (rc != HC_OK) return rc;
+  if (!out || !same_shape(in, out) || !is_c_contiguous(out))
+    return fail(HC_ERR_VALUE, "output must be C-contiguous with the image's shape");
+  if (out->dtype != HC_FLOAT32 && out->dtype != HC_FLOAT64)
+    return fail(HC_ERR_VALUE, "frangi output must be float32 or float64");
+  if (mode < HC_MODE_REFLECT || mode > HC_MODE_CONSTANT)
+    return fail(HC_ERR_VALUE, "boundary mode not supported");
+  if (nsig < 0 || (nsig > 0 && (!taps_s || !taps_d))) return fail(HC_ERR_VALUE, "bad sigmas");
+  if (!work) return fail(HC_ERR_VALUE, "null work arrays");
+  const bool f64 = out->dtype == HC_FLOAT64;
+  for (int k = 0; k < nsig * in->ndim; ++k)
+    if ((taps_s[k].w_f64 != 0) != f64 || (taps_d[k].w_f64 != 0) != f64)
+      return fail(HC_ERR_VALUE, "weights dtype must match the work dtype");
+  hipStream_t st = (hipStream_t)stream;
+  return f64 ? frangi_impl<double>(in, out, taps_s, taps_d, nsig, alpha, beta, gamma,
+                                   black_ridges, mode, cval, work, nwork, scratch, st)
+             : frangi_impl<float>(in, out, taps_s, taps_d, nsig, alpha, beta, gamma,
+                                  black_ridges, mode, cval, work, nwork, scratch, st);
+}
+
+}  // extern "C"

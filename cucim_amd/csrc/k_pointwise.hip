@@ -1,0 +1,550 @@
+// Pointwise and small-stencil kernels of the path: rgb2gray, img_as_float
+// conversion, finite-difference gradient, the fused 2-D Sobel/Prewitt
+// magnitude, mask erosion, running maximum and strided copy-with-cast.
+//
+// Reference (paths relative to python/cucim/src/cucim/skimage/):
+//   color/colorconv.py:1037-1095, :160-180   rgb2gray
+//   util/dtype.py:219-377                    _convert (int -> float)
+//   _shared/_gradient.py                     gradient
+//   filters/edges.py:75-85, 148-241          _mask_filter_result, _generic_edge_filter
+#include <algorithm>
+#include <cstring>
+
+#include "hc_common.h"
+
+namespace hc {
+
+namespace {
+int64_t grid_for(int64_t n) { return std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 256 * 64)); }
+
+double imax_of(int dt) {
+  switch (dt) {
+    case HC_UINT8: return 255.0;
+    case HC_INT8: return 127.0;
+    case HC_UINT16: return 65535.0;
+    case HC_INT16: return 32767.0;
+    case HC_UINT32: return 4294967295.0;
+    case HC_INT32: return 2147483647.0;
+    case HC_UINT64: return 18446744073709551615.0;
+    case HC_INT64: return 9223372036854775807.0;
+    default: return 1.0;
+  }
+}
+bool is_int(int dt) { return dt >= HC_INT8 && dt <= HC_UINT64; }
+bool is_signed_int(int dt) {
+  return dt == HC_INT8 || dt == HC_INT16 || dt == HC_INT32 || dt == HC_INT64;
+}
+}  // namespace
+
+// Separately rounded multiply / add: the reference forms these in separate
+// kernels (square in place, then accumulate), so they must not contract.
+__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ float add_rn(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ double add_rn(double a, double b) { return __dadd_rn(a, b); }
+
+// one tap, skipped when the weight is zero (as the reference skips it)
+template <typename F>
+__device__ __forceinline__ F tap(F acc, F x, F w) { return nz(w) ? fma_(x, w, acc) : acc; }
+
+// img_as_float-style conversion of one element to the computation type C:
+// uint -> C(x) * C(1/imax); int -> max(that, -1); bool/float -> C(x).
+template <typename C>
+__device__ __forceinline__ C to_float_elem(const void* in, int64_t i, int dt, C scale, int sgn) {
+  C v = load_as<C>(in, i, dt);
+  if (scale != (C)0) {
+    v = v * scale;
+    if (sgn) v = v < (C)-1 ? (C)-1 : v;
+  }
+  return v;
+}
+
+// ------------------------------------------------------------- rgb2gray --
+// gray = 0.2125 R + 0.7154 G + 0.0721 B with double literals, summed left to
+// right in double without contraction, stored in the output float type.
+template <typename C>
+__global__ __launch_bounds__(256) void k_rgb2gray(const void* __restrict__ in, int dt,
+                                                  void* __restrict__ out, int out_dt, int64_t n,
+                                                  C scale, int sgn) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    const C r = to_float_elem<C>(in, 3 * i, dt, scale, sgn);
+    const C g = to_float_elem<C>(in, 3 * i + 1, dt, scale, sgn);
+    const C b = to_float_elem<C>(in, 3 * i + 2, dt, scale, sgn);
+    // computation-type values are first rounded to the float image type
+    double rd, gd, bd;
+    if (out_dt == HC_FLOAT32) {
+      rd = (double)(float)r; gd = (double)(float)g; bd = (double)(float)b;
+    } else {
+      rd = (double)r; gd = (double)g; bd = (double)b;
+    }
+    double y = __dmul_rn(0.2125, rd);
+    y = __dadd_rn(y, __dmul_rn(0.7154, gd));
+    y = __dadd_rn(y, __dmul_rn(0.0721, bd));
+    if (out_dt == HC_FLOAT32) ((float*)out)[i] = (float)y;
+    else ((double*)out)[i] = y;
+  }
+}
+
+template <typename C>
+__global__ __launch_bounds__(256) void k_to_float(const void* __restrict__ in, int dt,
+                                                  void* __restrict__ out, int out_dt, int64_t n,
+                                                  C scale, int sgn) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    const C v = to_float_elem<C>(in, i, dt, scale, sgn);
+    if (out_dt == HC_FLOAT32) ((float*)out)[i] = (float)v;
+    else ((double*)out)[i] = (double)v;
+  }
+}
+
+// ------------------------------------------------------------- gradient --
+template <typename F>
+__global__ __launch_bounds__(256) void k_gradient(const F* __restrict__ f, F* __restrict__ o,
+                                                  int64_t n, int64_t N, int64_t inner) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    const int64_t p = (i / inner) % N;
+    F v;
+    if (p == 0) v = f[i + inner] - f[i];
+    else if (p == N - 1) v = f[i] - f[i - inner];
+    else v = (f[i + inner] - f[i - inner]) / (F)2.0;
+    o[i] = v;
+  }
+}
+
+// ------------------------------------------------------ edge magnitude --
+// 2-D _generic_edge_filter with axis=None: for edge axis 0 convolve with the
+// edge kernel along axis 0 then the smoothing kernel along axis 1, square;
+// the same with the axes swapped; add; sqrt.  Every 1-D pass result is
+// rounded to F (separate arrays upstream).  cval is not forwarded upstream,
+// so constant mode pads with 0.  Weights are correlation weights (already
+// reversed from the convolution weights).
+constexpr int kEdgeTX = 64, kEdgeTY = 16;
+
+template <typename F>
+struct EdgeArgs {
+  const void* in;
+  int in_dt;
+  F* out;
+  int64_t H, Wd;
+  int mode;
+  F scale;
+  int sgn;
+  int nty, ntx;
+  F e[3], s[3];  // correlation weights
+};
+
+template <typename F>
+__global__ __launch_bounds__(256) void k_edge2d(const EdgeArgs<F> a) {
+  __shared__ F T[kEdgeTY + 2][kEdgeTX + 3];
+  __shared__ F EY[kEdgeTY][kEdgeTX + 3];  // edge along axis 0, cols -1..TX
+  __shared__ F EX[kEdgeTY + 2][kEdgeTX + 1];  // edge along axis 1, rows -1..TY
+  const int64_t tile = blockIdx.x;
+  const int64_t ty = tile / a.ntx, tx = tile - ty * a.ntx;
+  const int64_t y0 = ty * kEdgeTY, x0 = tx * kEdgeTX;
+  const int tid = threadIdx.x;
+  const bool cst = a.mode == HC_MODE_CONSTANT;
+  // image tile with one halo row/column on each side
+  for (int e = tid; e < (kEdgeTY + 2) * (kEdgeTX + 2); e += 256) {
+    const int r = e / (kEdgeTX + 2), c = e - r * (kEdgeTX + 2);
+    const int64_t y = remap(y0 + r - 1, a.H, a.mode), x = remap(x0 + c - 1, a.Wd, a.mode);
+    T[r][c] = (y < 0 || x < 0) ? (F)0 : to_float_elem<F>(a.in, y * a.Wd + x, a.in_dt, a.scale, a.sgn);
+  }
+  __syncthreads();
+  // first passes (edge kernel)
+  for (int e = tid; e < kEdgeTY * (kEdgeTX + 2); e += 256) {
+    const int r = e / (kEdgeTX + 2), c = e - r * (kEdgeTX + 2);
+    F v = (F)0;
+    v = tap(v, T[r][c], a.e[0]);
+    v = tap(v, T[r + 1][c], a.e[1]);
+    v = tap(v, T[r + 2][c], a.e[2]);
+    const int64_t x = x0 + c - 1;
+    if (cst && (x < 0 || x >= a.Wd)) v = (F)0;
+    EY[r][c] = v;
+  }
+  for (int e = tid; e < (kEdgeTY + 2) * kEdgeTX; e += 256) {
+    const int r = e / kEdgeTX, c = e - r * kEdgeTX;
+    F v = (F)0;
+    v = tap(v, T[r][c], a.e[0]);
+    v = tap(v, T[r][c + 1], a.e[1]);
+    v = tap(v, T[r][c + 2], a.e[2]);
+    const int64_t y = y0 + r - 1;
+    if (cst && (y < 0 || y >= a.H)) v = (F)0;
+    EX[r][c] = v;
+  }
+  __syncthreads();
+  for (int e = tid; e < kEdgeTY * kEdgeTX; e += 256) {
+    const int r = e / kEdgeTX, c = e - r * kEdgeTX;
+    const int64_t y = y0 + r, x = x0 + c;
+    if (y >= a.H || x >= a.Wd) continue;
+    F A = (F)0, B = (F)0;
+    A = tap(A, EY[r][c], a.s[0]);
+    A = tap(A, EY[r][c + 1], a.s[1]);
+    A = tap(A, EY[r][c + 2], a.s[2]);
+    B = tap(B, EX[r][c], a.s[0]);
+    B = tap(B, EX[r + 1][c], a.s[1]);
+    B = tap(B, EX[r + 2][c], a.s[2]);
+    a.out[y * a.Wd + x] = sqrt(add_rn(mul_rn(A, A), mul_rn(B, B)));
+  }
+}
+
+// ------------------------------------------------------ mask erosion --
+template <typename F>
+__global__ __launch_bounds__(256) void k_mask_eroded(F* __restrict__ out,
+                                                     const uint8_t* __restrict__ mask, int64_t n,
+                                                     int ndim, int64_t sh0, int64_t sh1,
+                                                     int64_t sh2, int64_t sh3) {
+  const int64_t sh[4] = {sh0, sh1, sh2, sh3};
+  int nb = 1;
+  for (int d = 0; d < ndim; ++d) nb *= 3;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    int64_t pos[4] = {0, 0, 0, 0};
+    int64_t rem = i;
+    for (int d = ndim - 1; d >= 0; --d) {
+      pos[d] = rem % sh[d];
+      rem /= sh[d];
+    }
+    bool keep = true;
+    for (int k = 0; k < nb && keep; ++k) {
+      int kk = k;
+      int64_t off = 0;
+      for (int d = ndim - 1; d >= 0; --d) {
+        const int64_t q = pos[d] + (kk % 3) - 1;
+        kk /= 3;
+        if (q < 0 || q >= sh[d]) { keep = false; break; }
+        int64_t stride = 1;
+        for (int e = d + 1; e < ndim; ++e) stride *= sh[e];
+        off += q * stride;
+      }
+      if (keep && mask[off] == 0) keep = false;
+    }
+    out[i] = out[i] * (keep ? (F)1 : (F)0);
+  }
+}
+
+template <typename F>
+__global__ __launch_bounds__(256) void k_maximum(F* __restrict__ a, const F* __restrict__ b, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    const F x = a[i], y = b[i];
+    a[i] = (x != x) ? x : ((y != y) ? y : (x > y ? x : y));
+  }
+}
+
+template <typename F>
+__global__ __launch_bounds__(256) void k_square_acc(F* __restrict__ acc, const F* __restrict__ v,
+                                                    int64_t n, int first) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    const F s = mul_rn(v[i], v[i]);
+    acc[i] = first ? s : add_rn(acc[i], s);
+  }
+}
+
+template <typename F>
+__global__ __launch_bounds__(256) void k_sqrt(F* __restrict__ a, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256)
+    a[i] = sqrt(a[i]);
+}
+
+struct CopyArgs {
+  const void* in;
+  void* out;
+  int in_dt, out_dt;
+  int ndim;
+  int64_t n;
+  int64_t shape[HC_MAX_NDIM];
+  int64_t istride[HC_MAX_NDIM];
+  int64_t ostride[HC_MAX_NDIM];
+};
+
+// kind: 0 = same dtype (raw element copy), 1 = integer/bool -> integer/bool
+// (C integer conversion, modular), 2 = through the float type W.
+__device__ __forceinline__ uint64_t load_bits(const void* p, int64_t i, int sz) {
+  switch (sz) {
+    case 1: return ((const uint8_t*)p)[i];
+    case 2: return ((const uint16_t*)p)[i];
+    case 4: return ((const uint32_t*)p)[i];
+    default: return ((const uint64_t*)p)[i];
+  }
+}
+__device__ __forceinline__ void store_bits(void* p, int64_t i, int sz, uint64_t v) {
+  switch (sz) {
+    case 1: ((uint8_t*)p)[i] = (uint8_t)v; return;
+    case 2: ((uint16_t*)p)[i] = (uint16_t)v; return;
+    case 4: ((uint32_t*)p)[i] = (uint32_t)v; return;
+    default: ((uint64_t*)p)[i] = v; return;
+  }
+}
+__device__ __forceinline__ int64_t load_int(const void* p, int64_t i, int dt) {
+  switch (dt) {
+    case HC_BOOL: return ((const uint8_t*)p)[i] != 0;
+    case HC_INT8: return ((const int8_t*)p)[i];
+    case HC_UINT8: return ((const uint8_t*)p)[i];
+    case HC_INT16: return ((const int16_t*)p)[i];
+    case HC_UINT16: return ((const uint16_t*)p)[i];
+    case HC_INT32: return ((const int32_t*)p)[i];
+    case HC_UINT32: return ((const uint32_t*)p)[i];
+    default: return ((const int64_t*)p)[i];  // int64 / uint64 bit pattern
+  }
+}
+
+template <typename W>
+__global__ __launch_bounds__(256) void k_copy_cast(const CopyArgs a, int kind, int isz, int osz) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < a.n;
+       i += (int64_t)gridDim.x * 256) {
+    int64_t rem = i, io = 0, oo = 0;
+#pragma unroll
+    for (int d = HC_MAX_NDIM - 1; d >= 0; --d) {
+      if (d < a.ndim) {
+        const int64_t p = rem % a.shape[d];
+        rem /= a.shape[d];
+        io += p * a.istride[d];
+        oo += p * a.ostride[d];
+      }
+    }
+    if (kind == 0) {
+      store_bits(a.out, oo, osz, load_bits(a.in, io, isz));
+    } else if (kind == 1) {
+      const int64_t v = load_int(a.in, io, a.in_dt);
+      if (a.out_dt == HC_BOOL) store_bits(a.out, oo, 1, v != 0);
+      else store_bits(a.out, oo, osz, (uint64_t)v);
+    } else {
+      store_cast<W>(a.out, oo, a.out_dt, load_as<W>(a.in, io, a.in_dt));
+    }
+  }
+}
+
+}  // namespace hc
+
+using namespace hc;
+
+extern "C" {
+
+int hc_rgb2gray(const hc_array* in, const hc_array* out, void* stream) {
+  if (!in || !out) return fail(HC_ERR_VALUE, "null array");
+  if (in->ndim < 1 || in->shape[in->ndim - 1] != 3)
+    return fail(HC_ERR_VALUE, "the input array must have size 3 along `channel_axis`");
+  if (!is_c_contiguous(in) || !is_c_contiguous(out))
+    return fail(HC_ERR_VALUE, "arrays must be C-contiguous");
+  if (out->dtype != HC_FLOAT32 && out->dtype != HC_FLOAT64)
+    return fail(HC_ERR_VALUE, "output must be float32 or float64");
+  if (out->ndim != in->ndim - 1) return fail(HC_ERR_VALUE, "output shape not correct");
+  for (int d = 0; d < out->ndim; ++d)
+    if (out->shape[d] != in->shape[d]) return fail(HC_ERR_VALUE, "output shape not correct");
+  const int64_t n = numel(out);
+  if (n == 0) return HC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int dt = in->dtype;
+  const int sgn = is_signed_int(dt);
+  // computation type: first of (out, float32, float64) at least as wide as the input
+  const int isz = dtype_size(dt);
+  const bool c64 = is_int(dt) ? (out->dtype == HC_FLOAT64 || isz > 4) : (dt == HC_FLOAT64 || out->dtype == HC_FLOAT64);
+  if (c64) {
+    const double sc = is_int(dt) ? 1.0 / imax_of(dt) : 0.0;
+    HC_LAUNCH("k_rgb2gray", st, k_rgb2gray<double>, dim3((unsigned)grid_for(n)), dim3(256), 0, in->data, dt, out->data, out->dtype, n, sc, sgn);
+  } else {
+    const float sc = is_int(dt) ? (float)(1.0 / imax_of(dt)) : 0.0f;
+    HC_LAUNCH("k_rgb2gray", st, k_rgb2gray<float>, dim3((unsigned)grid_for(n)), dim3(256), 0, in->data, dt, out->data, out->dtype, n, sc, sgn);
+  }
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+int hc_to_float(const hc_array* in, const hc_array* out, void* stream) {
+  if (!in || !out) return fail(HC_ERR_VALUE, "null array");
+  if (!same_shape(in, out) || !is_c_contiguous(in) || !is_c_contiguous(out))
+    return fail(HC_ERR_VALUE, "arrays must be C-contiguous with equal shapes");
+  if (out->dtype != HC_FLOAT32 && out->dtype != HC_FLOAT64)
+    return fail(HC_ERR_VALUE, "output must be float32 or float64");
+  const int64_t n = numel(in);
+  if (n == 0) return HC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int dt = in->dtype;
+  const int sgn = is_signed_int(dt);
+  const int isz = dtype_size(dt);
+  const bool c64 = is_int(dt) ? (out->dtype == HC_FLOAT64 || isz > 4) : (out->dtype == HC_FLOAT64 || dt == HC_FLOAT64);
+  if (c64) {
+    const double sc = is_int(dt) ? 1.0 / imax_of(dt) : 0.0;
+    HC_LAUNCH("k_to_float", st, k_to_float<double>, dim3((unsigned)grid_for(n)), dim3(256), 0, in->data, dt, out->data, out->dtype, n, sc, sgn);
+  } else {
+    const float sc = is_int(dt) ? (float)(1.0 / imax_of(dt)) : 0.0f;
+    HC_LAUNCH("k_to_float", st, k_to_float<float>, dim3((unsigned)grid_for(n)), dim3(256), 0, in->data, dt, out->data, out->dtype, n, sc, sgn);
+  }
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+int hc_gradient(const hc_array* in, const hc_array* out, int axis, void* stream) {
+  if (!in || !out) return fail(HC_ERR_VALUE, "null array");
+  if (!same_shape(in, out) || !is_c_contiguous(in) || !is_c_contiguous(out) ||
+      in->dtype != out->dtype)
+    return fail(HC_ERR_VALUE, "arrays must be C-contiguous with equal shapes and dtypes");
+  if (in->dtype != HC_FLOAT32 && in->dtype != HC_FLOAT64)
+    return fail(HC_ERR_VALUE, "gradient needs float32/float64");
+  if (axis < 0) axis += in->ndim;
+  if (axis < 0 || axis >= in->ndim) return fail(HC_ERR_VALUE, "invalid axis");
+  if (in->shape[axis] < 2)
+    return fail(HC_ERR_VALUE,
+                "Shape of array too small to calculate a numerical gradient, at least 2 "
+                "elements are required.");
+  const Lines ln = lines_of(in, axis);
+  const int64_t n = numel(in);
+  if (n == 0) return HC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (in->dtype == HC_FLOAT64)
+    HC_LAUNCH("k_gradient", st, k_gradient<double>, dim3((unsigned)grid_for(n)), dim3(256), 0, (const double*)in->data, (double*)out->data, n, ln.N, ln.inner);
+  else
+    HC_LAUNCH("k_gradient", st, k_gradient<float>, dim3((unsigned)grid_for(n)), dim3(256), 0, (const float*)in->data, (float*)out->data, n, ln.N, ln.inner);
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+int hc_edge_magnitude(const hc_array* in, const hc_array* out, const double* edge3,
+                      const double* smooth3, int axis_mask, int mode, const hc_array* work,
+                      void* stream) {
+  (void)work;
+  if (!in || !out || !edge3 || !smooth3) return fail(HC_ERR_VALUE, "null argument");
+  if (in->ndim != 2 || axis_mask != 3)
+    return fail(HC_ERR_UNSUPPORTED, "fused edge magnitude handles 2-D inputs over both axes");
+  if (!same_shape(in, out) || !is_c_contiguous(in) || !is_c_contiguous(out))
+    return fail(HC_ERR_VALUE, "arrays must be C-contiguous with equal shapes");
+  if (out->dtype != HC_FLOAT32 && out->dtype != HC_FLOAT64)
+    return fail(HC_ERR_VALUE, "output must be float32 or float64");
+  if (mode < HC_MODE_REFLECT || mode > HC_MODE_CONSTANT)
+    return fail(HC_ERR_VALUE, "boundary mode not supported");
+  const int64_t H = in->shape[0], Wd = in->shape[1];
+  if (H == 0 || Wd == 0) return HC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int dt = in->dtype;
+  const int ntx = (int)((Wd + kEdgeTX - 1) / kEdgeTX), nty = (int)((H + kEdgeTY - 1) / kEdgeTY);
+  const int64_t ntiles = (int64_t)ntx * nty;
+  if (out->dtype == HC_FLOAT64) {
+    EdgeArgs<double> a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = in->data; a.in_dt = dt; a.out = (double*)out->data; a.H = H; a.Wd = Wd;
+    a.mode = mode; a.scale = is_int(dt) ? 1.0 / imax_of(dt) : 0.0; a.sgn = is_signed_int(dt);
+    a.ntx = ntx; a.nty = nty;
+    for (int k = 0; k < 3; ++k) { a.e[k] = edge3[k]; a.s[k] = smooth3[k]; }
+    HC_LAUNCH("k_edge2d", st, k_edge2d<double>, dim3((unsigned)ntiles), dim3(256), 0, a);
+  } else {
+    EdgeArgs<float> a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = in->data; a.in_dt = dt; a.out = (float*)out->data; a.H = H; a.Wd = Wd;
+    a.mode = mode; a.scale = is_int(dt) ? (float)(1.0 / imax_of(dt)) : 0.0f;
+    a.sgn = is_signed_int(dt);
+    a.ntx = ntx; a.nty = nty;
+    for (int k = 0; k < 3; ++k) { a.e[k] = (float)edge3[k]; a.s[k] = (float)smooth3[k]; }
+    HC_LAUNCH("k_edge2d", st, k_edge2d<float>, dim3((unsigned)ntiles), dim3(256), 0, a);
+  }
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+int hc_mask_eroded(const hc_array* out, const hc_array* mask, void* stream) {
+  if (!out || !mask) return fail(HC_ERR_VALUE, "null array");
+  if (!same_shape(out, mask) || !is_c_contiguous(out) || !is_c_contiguous(mask))
+    return fail(HC_ERR_VALUE, "mask must be C-contiguous with the output's shape");
+  if (mask->dtype != HC_BOOL && mask->dtype != HC_UINT8)
+    return fail(HC_ERR_VALUE, "mask must be bool or uint8");
+  if (out->ndim > 4) return fail(HC_ERR_UNSUPPORTED, "mask erosion supports up to 4-D");
+  const int64_t n = numel(out);
+  if (n == 0) return HC_OK;
+  int64_t sh[4] = {1, 1, 1, 1};
+  for (int d = 0; d < out->ndim; ++d) sh[d] = out->shape[d];
+  hipStream_t st = (hipStream_t)stream;
+  if (out->dtype == HC_FLOAT64)
+    HC_LAUNCH("k_mask_eroded", st, k_mask_eroded<double>, dim3((unsigned)grid_for(n)), dim3(256), 0, (double*)out->data, (const uint8_t*)mask->data, n, out->ndim, sh[0], sh[1], sh[2], sh[3]);
+  else if (out->dtype == HC_FLOAT32)
+    HC_LAUNCH("k_mask_eroded", st, k_mask_eroded<float>, dim3((unsigned)grid_for(n)), dim3(256), 0, (float*)out->data, (const uint8_t*)mask->data, n, out->ndim, sh[0], sh[1], sh[2], sh[3]);
+  else
+    return fail(HC_ERR_VALUE, "output must be float32 or float64");
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+int hc_maximum_inplace(const hc_array* a, const hc_array* b, void* stream) {
+  if (!a || !b) return fail(HC_ERR_VALUE, "null array");
+  if (!same_shape(a, b) || !is_c_contiguous(a) || !is_c_contiguous(b) || a->dtype != b->dtype)
+    return fail(HC_ERR_VALUE, "arrays must be C-contiguous with equal shapes and dtypes");
+  const int64_t n = numel(a);
+  if (n == 0) return HC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (a->dtype == HC_FLOAT64)
+    HC_LAUNCH("k_maximum", st, k_maximum<double>, dim3((unsigned)grid_for(n)), dim3(256), 0, (double*)a->data, (const double*)b->data, n);
+  else if (a->dtype == HC_FLOAT32)
+    HC_LAUNCH("k_maximum", st, k_maximum<float>, dim3((unsigned)grid_for(n)), dim3(256), 0, (float*)a->data, (const float*)b->data, n);
+  else
+    return fail(HC_ERR_VALUE, "maximum needs float32/float64");
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+int hc_square_acc(const hc_array* acc, const hc_array* v, int first, void* stream) {
+  if (!acc || !v) return fail(HC_ERR_VALUE, "null array");
+  if (!same_shape(acc, v) || !is_c_contiguous(acc) || !is_c_contiguous(v) || acc->dtype != v->dtype)
+    return fail(HC_ERR_VALUE, "arrays must be C-contiguous with equal shapes and dtypes");
+  const int64_t n = numel(acc);
+  if (n == 0) return HC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (acc->dtype == HC_FLOAT64)
+    HC_LAUNCH("k_square_acc", st, k_square_acc<double>, dim3((unsigned)grid_for(n)), dim3(256), 0, (double*)acc->data, (const double*)v->data, n, first);
+  else if (acc->dtype == HC_FLOAT32)
+    HC_LAUNCH("k_square_acc", st, k_square_acc<float>, dim3((unsigned)grid_for(n)), dim3(256), 0, (float*)acc->data, (const float*)v->data, n, first);
+  else
+    return fail(HC_ERR_VALUE, "square_acc needs float32/float64");
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+int hc_sqrt_inplace(const hc_array* a, void* stream) {
+  if (!a) return fail(HC_ERR_VALUE, "null array");
+  if (!is_c_contiguous(a)) return fail(HC_ERR_VALUE, "array must be C-contiguous");
+  const int64_t n = numel(a);
+  if (n == 0) return HC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (a->dtype == HC_FLOAT64)
+    HC_LAUNCH("k_sqrt", st, k_sqrt<double>, dim3((unsigned)grid_for(n)), dim3(256), 0, (double*)a->data, n);
+  else if (a->dtype == HC_FLOAT32)
+    HC_LAUNCH("k_sqrt", st, k_sqrt<float>, dim3((unsigned)grid_for(n)), dim3(256), 0, (float*)a->data, n);
+  else
+    return fail(HC_ERR_VALUE, "sqrt needs float32/float64");
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+int hc_copy_cast(const hc_array* in, const hc_array* out, void* stream) {
+  if (!in || !out) return fail(HC_ERR_VALUE, "null array");
+  if (!same_shape(in, out)) return fail(HC_ERR_VALUE, "shapes differ");
+  if (in->ndim > HC_MAX_NDIM) return fail(HC_ERR_VALUE, "unsupported ndim");
+  CopyArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = in->data;
+  a.out = out->data;
+  a.in_dt = in->dtype;
+  a.out_dt = out->dtype;
+  a.ndim = in->ndim;
+  a.n = numel(in);
+  for (int d = 0; d < in->ndim; ++d) {
+    a.shape[d] = in->shape[d];
+    a.istride[d] = in->strides[d];
+    a.ostride[d] = out->strides[d];
+  }
+  if (a.n == 0) return HC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const bool iin = in->dtype <= HC_UINT64, iout = out->dtype <= HC_UINT64;  // ints and bool
+  const int kind = in->dtype == out->dtype ? 0 : (iin && iout ? 1 : 2);
+  const int isz = dtype_size(in->dtype), osz = dtype_size(out->dtype);
+  const bool f64 = in->dtype == HC_FLOAT64 || out->dtype == HC_FLOAT64 || isz >= 4;
+  if (f64)
+    HC_LAUNCH("k_copy_cast", st, k_copy_cast<double>, dim3((unsigned)grid_for(a.n)), dim3(256), 0, a, kind, isz, osz);
+  else
+    HC_LAUNCH("k_copy_cast", st, k_copy_cast<float>, dim3((unsigned)grid_for(a.n)), dim3(256), 0, a, kind, isz, osz);
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,5 @@
+"""``cucim.skimage.filters`` subset on the MI355X kernels."""
+
+from .._shared.filters import gaussian  # noqa: F401
+from .edges import farid, prewitt, scharr, sobel  # noqa: F401
+from .ridges import frangi  # noqa: F401

@@ -1,0 +1,100 @@
+"""Frangi vesselness (``filters/ridges.py:322-476`` of the reference, paths
+relative to ``python/cucim/src/cucim/skimage/``)."""
+
+from warnings import warn
+
+import numpy as np
+import torch
+
+from ... import _array, _lib
+from .._shared.utils import _supported_float_type, astype, check_nD
+from .._vendored._ndimage_util import _mode_code
+from ..feature.corner import (
+    _hessian_sequential,
+    _symmetric_compute_eigenvalues,
+    _taps_arrays,
+    hessian_kernels,
+)
+
+__all__ = ["frangi"]
+
+
+class FrangiPlan:
+    """Device buffers and per-sigma kernels of one Frangi configuration.
+
+    Re-using a plan across calls of the same shape avoids re-allocating the
+    work arrays and re-deriving the kernels (the batched drivers and the
+    benchmark do this).
+    """
+
+    def __init__(self, shape, float_dtype, sigmas, mode="reflect"):
+        self.shape = tuple(shape)
+        self.ndim = len(shape)
+        self.dtype = np.dtype(float_dtype)
+        self.sigmas = list(sigmas)
+        self.mode = mode
+        nH = 6 if self.ndim == 3 else 3
+        self.work = [_array.empty(shape, self.dtype) for _ in range(nH)]
+        self.work_hc = _array.hc_arrays(self.work)
+        self.scratch = torch.zeros(64, dtype=torch.uint8, device=_array.device())
+        kern = []
+        for s in self.sigmas:
+            kern += hessian_kernels(s, self.ndim, self.dtype)
+        self.ts, self.td, self._keep = _taps_arrays(kern, self.dtype == np.float64)
+
+    def run(self, image, out, alpha=0.5, beta=0.5, gamma=None, black_ridges=True, cval=0):
+        """Fused schedule; returns an hc status code."""
+        x = image.contiguous()
+        return _lib.lib().hc_frangi(
+            _array.hc_array(x), _array.hc_array(out), self.ts, self.td, len(self.sigmas),
+            float(alpha), float(beta), -1.0 if gamma is None else float(gamma),
+            1 if black_ridges else 0, _mode_code(self.mode), float(cval), self.work_hc,
+            len(self.work), self.scratch.data_ptr(), _array.stream())
+
+
+def _frangi_sequential(image, sigmas, alpha, beta, gamma, black_ridges, mode, cval, out):
+    """Reference pass order per sigma (Hessian by 1-D passes, closed-form
+    eigenvalues, vesselness), for images the fused schedule cannot hold."""
+    float_dtype = _array.np_dtype(out.dtype)
+    img = astype(image, float_dtype)
+    if not black_ridges:
+        img = -img
+    scratch = torch.zeros(64, dtype=torch.uint8, device=_array.device())
+    lib = _lib.lib()
+    for i, sigma in enumerate(sigmas):
+        H = _hessian_sequential(img, sigma, mode, cval, "rc")
+        eig = _symmetric_compute_eigenvalues(H, sort="ascending", abs_sort=True)
+        E = [eig[k] for k in range(img.dim())]
+        rc = lib.hc_frangi_from_eigvals(
+            _array.hc_arrays(E), img.dim(), _array.hc_array(out), float(alpha), float(beta),
+            -1.0 if gamma is None else float(gamma), 1 if i == 0 else 0,
+            scratch.data_ptr(), _array.stream())
+        _lib.check(rc, "frangi")
+    return out
+
+
+def frangi(image, sigmas=range(1, 10, 2), scale_range=None, scale_step=None, alpha=0.5,
+           beta=0.5, gamma=None, black_ridges=True, mode="reflect", cval=0):
+    """Frangi vesselness filter (2-D / 3-D), maximum over ``sigmas``."""
+    if scale_range is not None and scale_step is not None:
+        warn("Use keyword parameter `sigmas` instead of `scale_range` and "
+             "`scale_range` which will be removed in version 0.17.", stacklevel=2)
+        sigmas = np.arange(scale_range[0], scale_range[1], scale_step)
+    image = _array.as_tensor(image)
+    check_nD(image, [2, 3])
+    float_dtype = _supported_float_type(image.dtype)
+    sigmas = [float(s) for s in sigmas]
+    if not sigmas:
+        raise ValueError("sigmas must contain at least one value")
+    _mode_code(mode)
+    out = _array.empty(image.shape, float_dtype)
+    if _array.np_dtype(image.dtype) == np.float16:
+        image = astype(image, np.float32)
+    plan = FrangiPlan(image.shape, float_dtype, sigmas, mode)
+    rc = plan.run(image, out, alpha, beta, gamma, black_ridges, cval)
+    if rc == _lib.OK:
+        return out
+    if rc != _lib.ERR_RESOURCE:
+        _lib.check(rc, "frangi")
+    del plan
+    return _frangi_sequential(image, sigmas, alpha, beta, gamma, black_ridges, mode, cval, out)

@@ -1,0 +1,205 @@
+/*
+ * cucim_amd -- C ABI of the MI355X (gfx950) separable-filter / Hessian-ridge
+ * library.  Every entry point is stream-ordered, never throws, returns an
+ * hc_status code and leaves a message for hc_last_error() on failure.
+ *
+ * Arrays are described by hc_array (device pointer, element type, shape,
+ * element strides: the information a DLPack tensor carries).  The caller owns
+ * every buffer, including outputs and work arrays; the library allocates no
+ * device memory.  Weights are passed from the host as float64 values already
+ * rounded to the weights dtype, with a device copy for long kernels.
+ *
+ * The reference exposes this path only as Python functions (CuPy arrays in,
+ * CuPy arrays out); each entry point below names the reference function it
+ * replaces, paths relative to python/cucim/src/cucim/skimage/.
+ */
+#ifndef CUCIM_AMD_H
+#define CUCIM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HC_MAX_NDIM 8
+
+/* element types */
+typedef enum hc_dtype {
+  HC_BOOL = 0,
+  HC_INT8 = 1,
+  HC_UINT8 = 2,
+  HC_INT16 = 3,
+  HC_UINT16 = 4,
+  HC_INT32 = 5,
+  HC_UINT32 = 6,
+  HC_INT64 = 7,
+  HC_UINT64 = 8,
+  HC_FLOAT16 = 9,
+  HC_FLOAT32 = 10,
+  HC_FLOAT64 = 11
+} hc_dtype;
+
+/* boundary modes, _vendored/_ndimage_util.py:186-308 ('wrap' == 'grid-wrap',
+ * 'grid-mirror' == 'reflect', 'grid-constant' == 'constant') */
+typedef enum hc_mode {
+  HC_MODE_REFLECT = 0,
+  HC_MODE_MIRROR = 1,
+  HC_MODE_NEAREST = 2,
+  HC_MODE_WRAP = 3,
+  HC_MODE_CONSTANT = 4
+} hc_mode;
+
+/* line-kernel choice, the algorithm= keyword of
+ * _vendored/_ndimage_filters.py:135-221 */
+typedef enum hc_algorithm {
+  HC_ALGO_AUTO = 0,
+  HC_ALGO_TILED = 1,  /* 'shared_memory': LDS halo tiles */
+  HC_ALGO_GATHER = 2  /* 'elementwise': one work-item per output */
+} hc_algorithm;
+
+typedef enum hc_status {
+  HC_OK = 0,
+  HC_ERR_VALUE = 1,     /* bad argument (ValueError / RuntimeError upstream) */
+  HC_ERR_RESOURCE = 2,  /* LDS budget exceeded (ResourceLimitError upstream) */
+  HC_ERR_HIP = 3,       /* HIP runtime error */
+  HC_ERR_UNSUPPORTED = 4
+} hc_status;
+
+typedef struct hc_array {
+  void* data;                     /* device pointer */
+  int32_t dtype;                  /* hc_dtype */
+  int32_t ndim;
+  int64_t shape[HC_MAX_NDIM];
+  int64_t strides[HC_MAX_NDIM];   /* element strides */
+} hc_array;
+
+typedef struct hc_taps {
+  const double* w;   /* host weights, K values, rounded to the weights dtype */
+  int64_t K;
+  int32_t anchor;    /* K // 2 + origin: tap k reads position i + k - anchor */
+  int32_t w_f64;     /* weights / accumulation dtype: 0 float32, 1 float64 */
+  const void* w_dev; /* device copy (weights dtype); needed when
+                        K > hc_max_inline_taps(w_f64), else may be NULL */
+} hc_taps;
+
+const char* hc_last_error(void);
+int hc_version(void);
+int hc_max_inline_taps(int w_f64);
+/* largest line the fused Hessian / Frangi passes keep resident in LDS */
+int hc_max_resident_line(int w_f64);
+
+/* Launch tracing (no counterpart in the reference's skimage path, whose only
+ * tracing is NVTX in the I/O library).  While enabled, every kernel launch is
+ * bracketed by a hipEvent pair on its stream.  hc_timing_collect() waits for
+ * the recorded launches, writes up to max_records elapsed times (ms) and the
+ * kernel names ('\n'-separated) and clears the record. */
+int hc_timing_enable(int on);
+int hc_timing_collect(char* names, int names_len, double* ms, int max_records);
+
+/* correlate1d / convolve1d core: _vendored/_ndimage_filters.py:271-344
+ * (_correlate_or_convolve1d), the shared-memory kernel of
+ * filters/_separable_filtering.py:977-1070 and the elementwise kernel of
+ * _vendored/_ndimage_filters_core.py:247-408.  Convolution is expressed by
+ * the caller as correlation with reversed weights (_ndimage_filters.py:236-243).
+ * out must be C-contiguous with in's shape and must not overlap in. */
+int hc_correlate1d(const hc_array* in, const hc_array* out, const hc_taps* taps,
+                   int axis, int mode, double cval, int algorithm, void* stream);
+
+/* n-D correlate with a full weights array: _vendored/_ndimage_filters.py:224-257
+ * (_correlate_or_convolve) + _ndimage_filters_core.py:247-408.  w_dev holds
+ * prod(wshape) weights (weights dtype) in C order; offsets[d] = wshape[d]//2
+ * + origin[d]; modes per axis. */
+int hc_correlate_nd(const hc_array* in, const hc_array* out, const void* w_dev,
+                    const int64_t* wshape, const int32_t* offsets,
+                    const int32_t* modes, int w_f64, double cval, void* stream);
+
+/* Two 1-D passes of a 2-D array in one launch (axis 0, then axis 1), the
+ * intermediate rounded to out's dtype exactly as _run_1d_filters
+ * (_vendored/_ndimage_filters_core.py:124-163) stores it.  Serves
+ * gaussian_filter / uniform_filter / sobel / prewitt on 2-D inputs. */
+int hc_separable2d(const hc_array* in, const hc_array* out, const hc_taps* taps0,
+                   const hc_taps* taps1, int mode0, int mode1, double cval,
+                   void* stream);
+
+/* Hessian by Gaussian derivatives for 2-D / 3-D float32/float64 images:
+ * feature/corner.py:161-254 (_hessian_matrix_with_gaussian).
+ * taps_s[d] / taps_d[d]: order-0 / order-1 kernels of axis d.  H: the
+ * ndim*(ndim+1)/2 outputs (C-contiguous, image dtype) in
+ * combinations_with_replacement order; order_xy != 0 reverses the axes
+ * (2-D only).  No work memory is needed. */
+int hc_hessian_gaussian(const hc_array* in, const hc_array* H, int nH,
+                        const hc_taps* taps_s, const hc_taps* taps_d, int mode,
+                        double cval, int order_xy, void* stream);
+
+/* Closed-form symmetric 2x2 / 3x3 eigenvalues: feature/corner.py:473-696
+ * (_symmetric_compute_eigenvalues).  S: 3 or 6 element arrays, eig: ndim
+ * outputs.  sort: 0 ascending, 1 descending. */
+int hc_sym_eigvals(const hc_array* S, int nS, const hc_array* eig, int sort,
+                   int abs_sort, void* stream);
+
+/* Frangi vesselness: filters/ridges.py:322-476.  taps_s/taps_d hold
+ * nsig*ndim kernels (sigma-major).  gamma < 0 means "None" (gamma from the
+ * first sigma, computed on the device; no host synchronisation).  work: 6
+ * (3-D) or 3 (2-D) arrays of the image's shape in the work dtype;
+ * scratch: >= 64 bytes of device memory.  out: image shape, work dtype. */
+int hc_frangi(const hc_array* in, const hc_array* out, const hc_taps* taps_s,
+              const hc_taps* taps_d, int nsig, double alpha, double beta,
+              double gamma, int black_ridges, int mode, double cval,
+              const hc_array* work, int nwork, void* scratch, void* stream);
+
+/* Frangi vesselness of one sigma from eigenvalues already sorted ascending
+ * by |value| (filters/ridges.py:428-475 after the eigenvalue step): eig holds
+ * ndim arrays; first_sigma != 0 writes out, else out = max(out, v).  With
+ * gamma < 0, the first sigma reduces max(sum lambda^2) into scratch
+ * (>= 64 bytes) and later sigmas read gamma from there.  Serves images whose
+ * lines do not fit the fused hc_frangi schedule. */
+int hc_frangi_from_eigvals(const hc_array* eig, int ndim, const hc_array* out,
+                           double alpha, double beta, double gamma,
+                           int first_sigma, void* scratch, void* stream);
+
+/* rgb2gray: color/colorconv.py:1037-1095 (+ _prepare_colorarray :160-180).
+ * in: (..., 3) C-contiguous, any real dtype; out: (...) float32/float64. */
+int hc_rgb2gray(const hc_array* in, const hc_array* out, void* stream);
+
+/* img_as_float32/64 style conversion (util/dtype.py:219-377): unsigned ints
+ * scale by 1/imax, signed ints scale and clip at -1, bool/float cast.
+ * in and out C-contiguous, same shape. */
+int hc_to_float(const hc_array* in, const hc_array* out, void* stream);
+
+/* Finite-difference gradient along one axis: _shared/_gradient.py. */
+int hc_gradient(const hc_array* in, const hc_array* out, int axis, void* stream);
+
+/* Edge magnitude sqrt((S_1 E_0 I)^2 + (S_0 E_1 I)^2) of a 2-D image,
+ * filters/edges.py:148-241 (_generic_edge_filter with axis=None) in one
+ * launch.  edge3/smooth3 are CORRELATION weights (the convolution weights
+ * reversed; edge already divided by sqrt(2)); integer input is converted as
+ * img_as_float does; constant mode pads with 0 (cval is not forwarded
+ * upstream).  axis_mask must be 3; work is unused.  Other ranks are composed
+ * from hc_correlate1d + hc_square_acc + hc_sqrt_inplace by the caller. */
+int hc_edge_magnitude(const hc_array* in, const hc_array* out,
+                      const double* edge3, const double* smooth3, int axis_mask,
+                      int mode, const hc_array* work, void* stream);
+
+/* out = out * binary_erosion(mask, full structure, border 0)
+ * (filters/edges.py:75-85).  mask: bool/uint8 C-contiguous of out's shape. */
+int hc_mask_eroded(const hc_array* out, const hc_array* mask, void* stream);
+
+/* Elementwise NaN-propagating maximum a = max(a, b) (cp.maximum). */
+int hc_maximum_inplace(const hc_array* a, const hc_array* b, void* stream);
+
+/* acc = v*v (first != 0) or acc += v*v, float32/float64, C-contiguous. */
+int hc_square_acc(const hc_array* acc, const hc_array* v, int first, void* stream);
+
+/* a = sqrt(a), float32/float64, C-contiguous. */
+int hc_sqrt_inplace(const hc_array* a, void* stream);
+
+/* Strided copy with dtype cast (cast<> semantics of
+ * _ndimage_filters_core.py:227-244): out[...] = cast(in[...]). */
+int hc_copy_cast(const hc_array* in, const hc_array* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUCIM_AMD_H */

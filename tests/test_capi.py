@@ -1,0 +1,75 @@
+"""CPU checks of the C ABI: the library loads, exports every declared entry
+point, and rejects bad arguments before touching the GPU."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cucim_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    with open(os.path.join(ROOT, "include", "cucim_amd.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hc_\w+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert _declared() == sorted(_lib.ENTRY_POINTS)
+
+
+def test_library_exports_every_entry_point():
+    lib = _lib.load_library()
+    for name in _declared():
+        assert hasattr(lib, name), name
+    assert lib.hc_version() == 1
+    assert lib.hc_max_inline_taps(0) == 256
+    assert lib.hc_max_inline_taps(1) == 128
+    assert lib.hc_max_resident_line(0) > 256
+
+
+def _arr(shape, dtype=_lib.FLOAT32, data=0x1000):
+    a = _lib.HCArray()
+    a.data = data
+    a.dtype = dtype
+    a.ndim = len(shape)
+    stride = 1
+    for d in range(len(shape) - 1, -1, -1):
+        a.shape[d] = shape[d]
+        a.strides[d] = stride
+        stride *= shape[d]
+    return a
+
+
+def test_argument_errors_return_before_launch():
+    lib = _lib.load_library()
+    t, keep = _lib.make_taps(np.ones(3), 1, False)
+    a, b = _arr((4, 5)), _arr((4, 6))
+    assert lib.hc_correlate1d(ctypes.byref(a), ctypes.byref(b), ctypes.byref(t), 0, 0, 0.0, 0,
+                              None) == _lib.ERR_VALUE
+    assert b"shape" in lib.hc_last_error()
+    b = _arr((4, 5))
+    assert lib.hc_correlate1d(ctypes.byref(a), ctypes.byref(b), ctypes.byref(t), 0, 9, 0.0, 0,
+                              None) == _lib.ERR_VALUE
+    bad, keep2 = _lib.make_taps(np.ones(3), 3, False)
+    assert lib.hc_correlate1d(ctypes.byref(a), ctypes.byref(b), ctypes.byref(bad), 0, 0, 0.0, 0,
+                              None) == _lib.ERR_VALUE
+    assert b"origin" in lib.hc_last_error()
+    # long kernels cannot be forced onto the tiled kernels
+    big, keep3 = _lib.make_taps(np.ones(301), 150, False)
+    assert lib.hc_correlate1d(ctypes.byref(a), ctypes.byref(b), ctypes.byref(big), 0, 0, 0.0,
+                              _lib.ALGO_TILED, None) == _lib.ERR_RESOURCE
+    # the fused Hessian only takes 2-D / 3-D float images
+    img = _arr((4, 5, 6, 7))
+    assert lib.hc_hessian_gaussian(ctypes.byref(img), None, 0, None, None, 0, 0.0, 0,
+                                   None) == _lib.ERR_VALUE
+    with pytest.raises(ValueError):
+        _lib.check(_lib.ERR_VALUE)
+    with pytest.raises(_lib.ResourceLimitError):
+        _lib.check(_lib.ERR_RESOURCE)

@@ -1,0 +1,319 @@
+"""GPU parity of the skimage-level functions against the oracle.
+
+Tolerances (float32 data): gaussian bit-exact against the float32-FMA
+emulation; Hessian elements within 2e-6 * max|H| of the FP64 oracle;
+eigenvalues within 1e-5 * max|lambda| of the oracle's closed form on the
+same float32 input; Frangi within max-abs 2e-4 and rel-L2 1e-4 of the FP64
+oracle.  rgb2gray bit-exact (all 2^24 uint8 triples).
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ndimage as on
+from oracle import skimage as osk
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def _np(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def test_gaussian_docstring_kats(gpu, kats):
+    from cucim_amd.skimage.filters import gaussian
+
+    kat = kats["gaussian_delta3x3"]
+    a = np.zeros((3, 3))
+    a[1, 1] = 1
+    for case in kat["cases"]:
+        g = _np(gaussian(_t(a), sigma=case["sigma"], mode=case["mode"]))
+        exp = np.array([[case["corner"], case["edge"], case["corner"]],
+                        [case["edge"], case["center"], case["edge"]],
+                        [case["corner"], case["edge"], case["corner"]]])
+        np.testing.assert_allclose(g, exp, atol=0.5 * 10 ** -kat["decimals"])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.uint8, np.float16])
+@pytest.mark.parametrize("channel_axis", [None, -1, 0])
+def test_gaussian_policy(gpu, dtype, channel_axis):
+    from cucim_amd.skimage.filters import gaussian
+
+    rng = np.random.default_rng(1)
+    x = (rng.random((33, 20, 3)) * (255 if dtype == np.uint8 else 1)).astype(dtype)
+    got = _np(gaussian(_t(x), sigma=1.5, channel_axis=channel_axis, preserve_range=False))
+    exp = osk.gaussian(x, sigma=1.5, channel_axis=channel_axis)
+    assert got.dtype == exp.dtype
+    np.testing.assert_allclose(got, exp, rtol=2e-6 if got.dtype == np.float32 else 1e-12,
+                               atol=1e-6 if got.dtype == np.float32 else 1e-12)
+
+
+def test_gaussian_out_and_errors(gpu):
+    from cucim_amd.skimage.filters import gaussian
+
+    x = _t(np.random.default_rng(0).random((20, 30)).astype(np.float32))
+    out = torch.empty_like(x)
+    r = gaussian(x, 2.0, out=out)
+    assert r is out
+    np.testing.assert_array_equal(_np(out), _np(gaussian(x, 2.0)))
+    y = x.clone()
+    gaussian(y, 2.0, out=y)
+    np.testing.assert_array_equal(_np(y), _np(gaussian(x, 2.0)))
+    with pytest.raises(ValueError):
+        gaussian(x, -1.0)
+    with pytest.raises(ValueError):
+        gaussian(x, 1.0, out=torch.empty(x.shape, dtype=torch.int32, device="cuda"))
+
+
+def test_sobel_kats(gpu):
+    from cucim_amd.skimage.filters import sobel
+
+    i, j = np.mgrid[-5:6, -5:6]
+    for img, sel in [((i >= 0).astype(float), i), ((j >= 0).astype(float), j)]:
+        r = _np(sobel(_t(img))) * math.sqrt(2)
+        np.testing.assert_allclose(r[sel == 0], 1)
+        np.testing.assert_allclose(r[np.abs(sel) > 1], 0, atol=1e-12)
+    assert (_np(sobel(_t(np.zeros((10, 10))), _t(np.ones((10, 10), bool)))) == 0).all()
+    for dt in (np.float16, np.float32, np.float64):
+        r = sobel(_t(np.random.default_rng(0).random((10, 10)).astype(dt)),
+                  _t(np.zeros((10, 10), bool)))
+        assert (_np(r) == 0).all()
+        assert r.dtype == (torch.float64 if dt == np.float64 else torch.float32)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.uint8])
+@pytest.mark.parametrize("mode", ["reflect", "nearest", "constant", "wrap", "mirror"])
+@pytest.mark.parametrize("shape", [(37, 64), (5, 300), (9, 10, 11)])
+@pytest.mark.parametrize("axis", [None, 0, -1])
+def test_sobel_vs_oracle(gpu, dtype, mode, shape, axis):
+    from cucim_amd.skimage.filters import sobel
+
+    rng = np.random.default_rng(2)
+    x = (rng.random(shape) * (255 if dtype == np.uint8 else 1)).astype(dtype)
+    got = _np(sobel(_t(x), mode=mode, axis=axis))
+    exp = osk.sobel(x, mode=mode, axis=axis)
+    assert got.dtype == exp.dtype
+    tol = 3e-6 if got.dtype == np.float32 else 1e-12
+    np.testing.assert_allclose(got, exp, rtol=tol, atol=tol)
+
+
+def test_sobel_mask(gpu):
+    from cucim_amd.skimage.filters import sobel
+
+    rng = np.random.default_rng(3)
+    x = rng.random((30, 40))
+    m = rng.random((30, 40)) > 0.2
+    got = _np(sobel(_t(x), _t(m)))
+    exp = osk.sobel(x, m)
+    np.testing.assert_allclose(got, exp, rtol=1e-12, atol=1e-12)
+
+
+def test_rgb2gray_all_uint8_triples(gpu):
+    from cucim_amd.skimage.color import rgb2gray
+
+    v = np.arange(1 << 24, dtype=np.uint32)
+    rgb = np.stack([(v >> 16) & 255, (v >> 8) & 255, v & 255], -1).astype(np.uint8)
+    rgb = rgb.reshape(4096, 4096, 3)
+    got = _np(rgb2gray(_t(rgb)))
+    exp = osk.rgb2gray(rgb)
+    assert got.dtype == np.float32
+    np.testing.assert_array_equal(got.view(np.uint32), exp.view(np.uint32))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.uint16, np.int16])
+@pytest.mark.parametrize("channel_axis", [0, 1, -1])
+def test_rgb2gray(gpu, dtype, channel_axis, kats):
+    from cucim_amd.skimage.color import rgb2gray
+
+    rng = np.random.default_rng(4)
+    x = (rng.random((17, 3, 23)) * 1000).astype(dtype)
+    x = np.moveaxis(x, 1, channel_axis) if channel_axis != 1 else x
+    got = _np(rgb2gray(_t(x), channel_axis=channel_axis))
+    exp = osk.rgb2gray(x, channel_axis=channel_axis)
+    assert got.dtype == exp.dtype
+    np.testing.assert_array_equal(got, exp)
+    w = _np(rgb2gray(_t(np.ones((1, 1, 3))), channel_axis=-1))
+    np.testing.assert_allclose(w, kats["rgb2gray_white"]["expected"], atol=1e-6)
+    with pytest.raises(ValueError):
+        rgb2gray(_t(np.zeros((4, 4, 4))))
+
+
+def test_hessian_fd_kats(gpu, kats):
+    from cucim_amd.skimage.feature import hessian_matrix, hessian_matrix_eigvals
+
+    k = kats["hessian_fd_5x5"]
+    a = np.zeros((5, 5))
+    a[2, 2] = 4
+    Hrr, Hrc, Hcc = hessian_matrix(_t(a), sigma=0.1, order="rc", use_gaussian_derivatives=False)
+    for g, name in [(Hrr, "Hrr"), (Hrc, "Hrc"), (Hcc, "Hcc")]:
+        np.testing.assert_allclose(_np(g), k[name], atol=1.5e-6)
+    ke = kats["hessian_eigvals_5x5"]
+    l1, l2 = hessian_matrix_eigvals([Hrr, Hrc, Hcc])
+    np.testing.assert_allclose(_np(l1), ke["l1"], atol=1.5e-6)
+    np.testing.assert_allclose(_np(l2), ke["l2"], atol=1.5e-6)
+    cube = np.zeros((5, 5, 5))
+    cube[2, 2, 2] = 4
+    Hs = hessian_matrix(_t(cube), sigma=0.1, order="rc", use_gaussian_derivatives=False)
+    np.testing.assert_allclose(_np(Hs[2])[:, 2, :], kats["hessian_fd_3d_slice"]["H02_slice"],
+                               atol=1.5e-6)
+    with pytest.warns(FutureWarning):
+        hessian_matrix(_t(a), sigma=0.1)
+
+
+@pytest.mark.parametrize("shape", [(40, 33), (12, 17, 23), (64, 64, 64)])
+@pytest.mark.parametrize("sigma", [0.8, 1.0, 2.0, 3.5])
+@pytest.mark.parametrize("mode", ["reflect", "constant", "nearest", "wrap", "mirror"])
+@pytest.mark.parametrize("order", ["rc", "xy"])
+def test_hessian_gaussian_vs_fp64(gpu, shape, sigma, mode, order):
+    from cucim_amd.skimage.feature import hessian_matrix
+
+    if order == "xy" and len(shape) == 3:
+        pytest.skip("xy is 2-D only")
+    if shape == (64, 64, 64) and (mode not in ("reflect", "constant") or sigma == 0.8):
+        pytest.skip("large case: two modes")
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(shape).astype(np.float32)
+    got = [_np(h) for h in hessian_matrix(_t(x), sigma, mode=mode, cval=0.5, order=order,
+                                          use_gaussian_derivatives=True)]
+    import scipy.ndimage as sndi
+
+    exp = osk.hessian_matrix(x.astype(np.float64), sigma, mode=mode, cval=0.5, order=order,
+                             use_gaussian_derivatives=True, ndi=sndi)
+    for g, e in zip(got, exp):
+        assert g.dtype == np.float32
+        scale = np.abs(e).max()
+        assert np.abs(g - e).max() <= 2e-6 * scale + 1e-7, (np.abs(g - e).max(), scale)
+
+
+def test_hessian_gaussian_float64_tight(gpu):
+    from cucim_amd.skimage.feature import hessian_matrix
+
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((14, 15, 16))
+    got = [_np(h) for h in hessian_matrix(_t(x), 1.5, mode="reflect",
+                                          use_gaussian_derivatives=True)]
+    exp = osk.hessian_matrix(x, 1.5, mode="reflect", use_gaussian_derivatives=True)
+    for g, e in zip(got, exp):
+        np.testing.assert_allclose(g, e, rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("nd", [2, 3])
+def test_eigvals_vs_oracle(gpu, dtype, nd):
+    from cucim_amd.skimage.feature.corner import _symmetric_compute_eigenvalues
+
+    rng = np.random.default_rng(7)
+    n = 200000
+    els = [rng.standard_normal(n).astype(dtype) for _ in range(3 if nd == 2 else 6)]
+    for sort in ("ascending", "descending"):
+        for abs_sort in (False, True):
+            got = _np(_symmetric_compute_eigenvalues([_t(e) for e in els], sort, abs_sort))
+            exp = osk.symmetric_compute_eigenvalues(els, sort, abs_sort)
+            assert got.dtype == exp.dtype
+            tol = 1e-5 if dtype == np.float32 else 1e-12
+            np.testing.assert_allclose(got, exp, atol=tol * np.abs(exp).max(), rtol=0)
+
+
+def _tubes(n, seed, ntubes=6):
+    rng = np.random.default_rng(seed)
+    grid = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).astype(np.float64)
+    vol = np.zeros((n, n, n))
+    for _ in range(ntubes):
+        p0, p1 = rng.uniform(0, n, 3), rng.uniform(0, n, 3)
+        r = rng.uniform(1, 3)
+        d = p1 - p0
+        t = np.clip(((grid - p0) @ d) / (d @ d), 0, 1)
+        dist2 = ((grid - (p0 + t[..., None] * d)) ** 2).sum(-1)
+        vol = np.maximum(vol, np.exp(-dist2 / (2 * r * r)))
+    return (vol + rng.normal(0, 0.05, vol.shape)).astype(np.float32)
+
+
+@pytest.mark.parametrize("n", [24, 48])
+@pytest.mark.parametrize("mode", ["reflect", "constant", "nearest"])
+@pytest.mark.parametrize("gamma", [None, 0.7])
+def test_frangi3d_vs_fp64(gpu, n, mode, gamma):
+    from cucim_amd.skimage.filters import frangi
+
+    x = _tubes(n, seed=n)
+    sig = (1, 2, 3)
+    got = _np(frangi(_t(x), sigmas=sig, black_ridges=False, mode=mode, gamma=gamma))
+    import scipy.ndimage as sndi
+
+    exp = osk.frangi(x.astype(np.float64), sigmas=sig, black_ridges=False, mode=mode,
+                     gamma=gamma, ndi=sndi)
+    assert got.dtype == np.float32
+    err = np.abs(got - exp)
+    rel_l2 = np.linalg.norm(got - exp) / max(np.linalg.norm(exp), 1e-30)
+    assert err.max() <= 2e-4 and rel_l2 <= 1e-4, (err.max(), rel_l2)
+
+
+@pytest.mark.parametrize("shape", [(64, 80), (300, 257)])
+@pytest.mark.parametrize("black", [True, False])
+def test_frangi2d_vs_fp64(gpu, shape, black):
+    from cucim_amd.skimage.filters import frangi
+
+    rng = np.random.default_rng(3)
+    yy, xx = np.mgrid[0 : shape[0], 0 : shape[1]].astype(np.float64)
+    x = np.zeros(shape)
+    for _ in range(5):
+        p0 = rng.uniform(0, 1, 2) * shape
+        ang = rng.uniform(0, np.pi)
+        d = np.abs((yy - p0[0]) * np.cos(ang) - (xx - p0[1]) * np.sin(ang))
+        x = np.maximum(x, np.exp(-d**2 / (2 * rng.uniform(1, 3) ** 2)))
+    x = (x + rng.normal(0, 0.05, shape)).astype(np.float32)
+    got = _np(frangi(_t(x), sigmas=(1, 2, 3), black_ridges=black))
+    import scipy.ndimage as sndi
+
+    exp = osk.frangi(x.astype(np.float64), sigmas=(1, 2, 3), black_ridges=black, ndi=sndi)
+    err = np.abs(got - exp)
+    rel_l2 = np.linalg.norm(got - exp) / max(np.linalg.norm(exp), 1e-30)
+    assert err.max() <= 2e-4 and rel_l2 <= 1e-4, (err.max(), rel_l2)
+
+
+def test_frangi_float64_tight(gpu):
+    from cucim_amd.skimage.filters import frangi
+
+    x = _tubes(20, 9).astype(np.float64)
+    got = _np(frangi(_t(x), sigmas=(1, 2), black_ridges=False))
+    exp = osk.frangi(x, sigmas=(1, 2), black_ridges=False)
+    np.testing.assert_allclose(got, exp, rtol=1e-9, atol=1e-12)
+
+
+def test_frangi_invariants(gpu):
+    from cucim_amd.skimage.filters import frangi
+
+    for shape in [(3, 3), (3, 3, 5)]:
+        z = np.zeros(shape, np.uint8)
+        np.testing.assert_allclose(_np(frangi(_t(z), black_ridges=True)), 0, atol=1e-3)
+        np.testing.assert_allclose(_np(frangi(_t(255 - z), black_ridges=False)), 0, atol=1e-3)
+    x = _tubes(32, 11).astype(np.float64)
+    a = _np(frangi(_t(x), sigmas=(1, 2), black_ridges=False))
+    b = _np(frangi(_t(1.0 - x), sigmas=(1, 2), black_ridges=True))
+    np.testing.assert_allclose(a, b, atol=1e-10, rtol=1e-10)
+    c = _np(frangi(_t(10 * x), sigmas=(1, 2), black_ridges=False))
+    np.testing.assert_allclose(a, c, atol=1e-3)
+
+
+def test_frangi_fused_vs_reference_order(gpu):
+    """The fused per-axis schedule against the reference's pass order (both on
+    the GPU) at a size the oracle cannot reach quickly."""
+    from cucim_amd.skimage.filters import ridges
+
+    x = _t(_tubes(96, 13))
+    sig = [1.0, 2.0, 3.0]
+    out1 = torch.empty_like(x)
+    plan = ridges.FrangiPlan(x.shape, np.float32, sig)
+    assert plan.run(x, out1, black_ridges=False) == 0
+    out2 = torch.empty_like(x)
+    ridges._frangi_sequential(x, sig, 0.5, 0.5, None, False, "reflect", 0, out2)
+    a, b = _np(out1), _np(out2)
+    rel_l2 = np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel_l2 <= 1e-5 and np.abs(a - b).max() <= 1e-4, (rel_l2, np.abs(a - b).max())
