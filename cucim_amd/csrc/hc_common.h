@@ -67,6 +67,62 @@ __device__ __forceinline__ W load_as(const void* __restrict__ base, int64_t i, i
   }
 }
 
+// Typed load: the fast path of the staging loops when the element type is
+// already the accumulation type.
+template <typename W>
+__device__ __forceinline__ W load_w(const void* __restrict__ base, int64_t i) {
+  return reinterpret_cast<const W*>(base)[i];
+}
+
+template <typename W>
+__device__ __forceinline__ constexpr int w_dtype() {
+  return sizeof(W) == 4 ? HC_FLOAT32 : HC_FLOAT64;
+}
+
+// Number of global loads each lane keeps in flight while staging a tile in
+// LDS (loads are issued as a batch before their LDS stores).
+constexpr int kStageBatch = 8;
+
+// Stage `total` elements of a tile into LDS with kStageBatch global loads in
+// flight per lane.  map(e, slot) returns the global element index of tile
+// element e (or -1 for the constant-mode value) and sets its LDS slot (or -1
+// to skip it).  TYPED: the buffer holds W already.
+template <typename W, bool TYPED, class Map>
+__device__ __forceinline__ void stage_lds(W* __restrict__ lds, int total, int tid, int nthreads,
+                                          const void* __restrict__ in, int dt, W cval,
+                                          bool negate, Map map) {
+  for (int e0 = tid; e0 < total; e0 += nthreads * kStageBatch) {
+    W v[kStageBatch];
+    int slot[kStageBatch];
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int e = e0 + u * nthreads;
+      slot[u] = -1;
+      v[u] = cval;
+      if (e < total) {
+        const int64_t g = map(e, slot[u]);
+        if (g >= 0) {
+          const W x = TYPED ? load_w<W>(in, g) : load_as<W>(in, g, dt);
+          v[u] = negate ? -x : x;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u)
+      if (slot[u] >= 0) lds[slot[u]] = v[u];
+  }
+}
+
+template <typename W, class Map>
+__device__ __forceinline__ void stage_any(W* __restrict__ lds, int total, int tid, int nthreads,
+                                          const void* __restrict__ in, int dt, W cval,
+                                          bool negate, Map map) {
+  if (dt == w_dtype<W>())
+    stage_lds<W, true>(lds, total, tid, nthreads, in, dt, cval, negate, map);
+  else
+    stage_lds<W, false>(lds, total, tid, nthreads, in, dt, cval, negate, map);
+}
+
 // Saturating truncation toward zero (float -> integer conversion of the GPU;
 // NaN -> 0).
 template <typename T, typename W>

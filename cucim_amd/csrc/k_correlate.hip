@@ -142,15 +142,14 @@ __global__ __launch_bounds__(256) void k_tiled_contig(const TiledArgs<W> a) {
     s0 = (seg - line * a.nseg) * kSeg;
     lbase = line * a.N;
     const int64_t p0 = s0 - a.tp.anchor;
-    const bool interior = p0 >= 0 && p0 + L <= a.N;
-    if (interior) {
-      for (int q = lane; q < L; q += 64) ext[q] = load_as<W>(a.in, lbase + p0 + q, a.in_dt);
-    } else {
-      for (int q = lane; q < L; q += 64) {
-        const int64_t r = remap(p0 + q, a.N, a.mode);
-        ext[q] = r < 0 ? a.cval : load_as<W>(a.in, lbase + r, a.in_dt);
-      }
-    }
+    const int64_t N = a.N, lb = lbase;
+    const int mode = a.mode;
+    stage_any<W>(ext, L, lane, 64, a.in, a.in_dt, a.cval, false,
+                 [=](int e, int& slot) -> int64_t {
+                   slot = e;
+                   const int64_t r = remap(p0 + e, N, mode);
+                   return r < 0 ? -1 : lb + r;
+                 });
   }
   __syncthreads();
   if (!active) return;
@@ -182,18 +181,17 @@ __global__ __launch_bounds__(256) void k_tiled_strided(const TiledArgs<W> a) {
   const int64_t a0 = sg * kStrTA;
   const int R = kStrTA + a.tp.K;
   const int64_t p0 = a0 - a.tp.anchor;
-  const bool interior = p0 >= 0 && p0 + R <= a.N;
-  for (int q = wv; q < R; q += 4) {
-    W v = a.cval;
-    if (lok) {
-      if (interior) {
-        v = load_as<W>(a.in, lbase + (p0 + q) * a.inner, a.in_dt);
-      } else {
-        const int64_t r = remap(p0 + q, a.N, a.mode);
-        if (r >= 0) v = load_as<W>(a.in, lbase + r * a.inner, a.in_dt);
-      }
-    }
-    ext[q * kStrStride + lane] = v;
+  {
+    const int64_t N = a.N, inner = a.inner;
+    const int mode = a.mode;
+    stage_any<W>(ext, R * 64, threadIdx.x, 256, a.in, a.in_dt, a.cval, false,
+                 [=](int e, int& slot) -> int64_t {
+                   const int q = e >> 6, c = e & 63;  // c == lane
+                   slot = q * kStrStride + c;
+                   if (!lok) return -1;
+                   const int64_t r = remap(p0 + q, N, mode);
+                   return r < 0 ? -1 : lbase + r * inner;
+                 });
   }
   __syncthreads();
   if (!lok) return;
@@ -236,12 +234,16 @@ __global__ __launch_bounds__(256) void k_sep2d(const Sep2dArgs<W> a) {
     const int64_t ty = tile / a.ntx, tx = tile - ty * a.ntx;
     const int64_t y0 = ty * TY, x0 = tx * a.TX;
     // stage the input tile with both axes boundary-extended
-    for (int e = threadIdx.x; e < RY * kSepCols; e += 256) {
-      const int q = e / kSepCols, c = e - q * kSepCols;
-      if (c >= CX) continue;
-      const int64_t y = remap(y0 - a.t0.anchor + q, a.H, a.mode0);
-      const int64_t x = remap(x0 - a.t1.anchor + c, a.Wd, a.mode1);
-      A[e] = (y < 0 || x < 0) ? a.cval : load_as<W>(a.in, y * a.Wd + x, a.in_dt);
+    {
+      const int64_t H = a.H, Wd = a.Wd, ya = y0 - a.t0.anchor, xa = x0 - a.t1.anchor;
+      const int m0 = a.mode0, m1 = a.mode1;
+      stage_any<W>(A, RY * kSepCols, threadIdx.x, 256, a.in, a.in_dt, a.cval, false,
+                   [=](int e, int& slot) -> int64_t {
+                     const int q = e >> 7, c = e & (kSepCols - 1);
+                     slot = c < CX ? e : -1;
+                     const int64_t y = remap(ya + q, H, m0), x = remap(xa + c, Wd, m1);
+                     return (y < 0 || x < 0) ? -1 : y * Wd + x;
+                   });
     }
     __syncthreads();
     // pass 1 (axis 0) over every staged column, rounded to the output dtype

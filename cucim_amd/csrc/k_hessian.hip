@@ -77,19 +77,20 @@ __global__ __launch_bounds__(256) void k_chain_strided(const ChainArgs<W> a) {
     need_d |= jb.o[k].first == 1;
   }
   // 1. stage whole lines (+ halo) of the job's input
-  for (int e = tid; e < E * kChC; e += 256) {
-    const int q = e / kChC, c = e - q * kChC;
-    const int64_t L = L0 + c;
-    W v = a.cval;
-    if (L < a.nlines) {
-      const int64_t r = remap((int64_t)q - anc, a.N, a.mode);
-      if (r >= 0) {
-        const int64_t lo = L / a.inner, li = L - lo * a.inner;
-        v = load_as<W>(jb.in, lo * a.N * a.inner + li + r * a.inner, jb.in_dt);
-        if (a.negate) v = -v;
-      }
-    }
-    xin[q * kChS + c] = v;
+  {
+    const int64_t N = a.N, inner = a.inner, nlines = a.nlines;
+    const int mode = a.mode;
+    stage_any<W>(xin, E * kChC, tid, 256, jb.in, jb.in_dt, a.cval, a.negate != 0,
+                 [=](int e, int& slot) -> int64_t {
+                   const int q = e / kChC, c = e - q * kChC;
+                   slot = q * kChS + c;
+                   const int64_t L = L0 + c;
+                   if (L >= nlines) return -1;
+                   const int64_t r = remap((int64_t)q - anc, N, mode);
+                   if (r < 0) return -1;
+                   const int64_t lo = L / inner, li = L - lo * inner;
+                   return lo * N * inner + li + r * inner;
+                 });
   }
   __syncthreads();
   // 2. first passes over positions [0, Nr)
@@ -180,18 +181,18 @@ __global__ __launch_bounds__(256) void k_chain_contig(const ChainArgs<W> a) {
     need_s |= jb.o[k].first == 0;
     need_d |= jb.o[k].first == 1;
   }
-  for (int e = tid; e < R * E; e += 256) {
-    const int rr = e / E, q = e - rr * E;
-    const int64_t row = row0 + rr;
-    W v = a.cval;
-    if (row < a.nlines) {
-      const int64_t r = remap((int64_t)q - anc, a.N, a.mode);
-      if (r >= 0) {
-        v = load_as<W>(jb.in, row * a.N + r, jb.in_dt);
-        if (a.negate) v = -v;
-      }
-    }
-    xin[e] = v;
+  {
+    const int64_t N = a.N, nlines = a.nlines;
+    const int mode = a.mode;
+    stage_any<W>(xin, R * E, tid, 256, jb.in, jb.in_dt, a.cval, a.negate != 0,
+                 [=](int e, int& slot) -> int64_t {
+                   const int rr = e / E, q = e - rr * E;
+                   slot = e;
+                   const int64_t row = row0 + rr;
+                   if (row >= nlines) return -1;
+                   const int64_t r = remap((int64_t)q - anc, N, mode);
+                   return r < 0 ? -1 : row * N + r;
+                 });
   }
   __syncthreads();
   const int nq = Nr / 4;
@@ -313,15 +314,18 @@ __global__ __launch_bounds__(256) void k_final_contig(const FinalArgs<W> a) {
   W res[NIN][kFinIPT][4];
 #pragma unroll
   for (int j = 0; j < NIN; ++j) {
-    for (int e = tid; e < R * E; e += 256) {
-      const int rr = e / E, q = e - rr * E;
-      const int64_t row = row0 + rr;
-      W v = a.cval;
-      if (row < a.nrows) {
-        const int64_t r = remap((int64_t)q - anc, a.N, a.mode);
-        if (r >= 0) v = a.in[j][row * a.N + r];
-      }
-      xin[e] = v;
+    {
+      const int64_t N = a.N, nrows = a.nrows;
+      const int mode = a.mode;
+      stage_lds<W, true>(xin, R * E, tid, 256, a.in[j], w_dtype<W>(), a.cval, false,
+                         [=](int e, int& slot) -> int64_t {
+                           const int rr = e / E, q = e - rr * E;
+                           slot = e;
+                           const int64_t row = row0 + rr;
+                           if (row >= nrows) return -1;
+                           const int64_t r = remap((int64_t)q - anc, N, mode);
+                           return r < 0 ? -1 : row * N + r;
+                         });
     }
     __syncthreads();
     const TapPack<W>& t1 = a.first[j] ? a.D : a.S;
@@ -845,6 +849,17 @@ int frangi_eig_impl(const hc_array* eig, int nd, const hc_array* out, double alp
   return HC_OK;
 }
 
+// Passes along different axes commute for the index-remapping modes and for
+// zero padding; with mode='constant' and cval != 0 they do not (the padding
+// value is not scaled by the other axes' kernels), so the fused schedule,
+// which reorders the reference's passes, must not run.
+int check_commuting(int mode, double cval) {
+  if (mode == HC_MODE_CONSTANT && cval != 0.0)
+    return fail(HC_ERR_UNSUPPORTED,
+                "mode='constant' with cval != 0 needs the reference pass order");
+  return HC_OK;
+}
+
 int check_float_image(const hc_array* in) {
   if (!in) return fail(HC_ERR_VALUE, "null array");
   if (in->ndim != 2 && in->ndim != 3)
@@ -878,6 +893,8 @@ int hc_hessian_gaussian(const hc_array* in, const hc_array* H, int nH, const hc_
   if (order_xy && in->ndim != 2) return fail(HC_ERR_VALUE, "Order 'xy' is only supported for 2D images.");
   if (mode < HC_MODE_REFLECT || mode > HC_MODE_CONSTANT)
     return fail(HC_ERR_VALUE, "boundary mode not supported");
+  rc = check_commuting(mode, cval);
+  if (rc != HC_OK) return rc;
   if (!H || !taps_s || !taps_d) return fail(HC_ERR_VALUE, "null argument");
   for (int k = 0; k < nH; ++k) {
     if (!same_shape(&H[k], in) || !is_c_contiguous(&H[k]) || H[k].dtype != in->dtype)
@@ -952,6 +969,8 @@ int hc_frangi(const hc_array* in, const hc_array* out, const hc_taps* taps_s,
   if (mode < HC_MODE_REFLECT || mode > HC_MODE_CONSTANT)
     return fail(HC_ERR_VALUE, "boundary mode not supported");
   if (nsig < 0 || (nsig > 0 && (!taps_s || !taps_d))) return fail(HC_ERR_VALUE, "bad sigmas");
+  rc = check_commuting(mode, cval);
+  if (rc != HC_OK) return rc;
   if (!work) return fail(HC_ERR_VALUE, "null work arrays");
   const bool f64 = out->dtype == HC_FLOAT64;
   for (int k = 0; k < nsig * in->ndim; ++k)

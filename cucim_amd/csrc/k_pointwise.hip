@@ -37,11 +37,25 @@ bool is_signed_int(int dt) {
 }  // namespace
 
 // Separately rounded multiply / add: the reference forms these in separate
-// kernels (square in place, then accumulate), so they must not contract.
-__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
-__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
-__device__ __forceinline__ float add_rn(float a, float b) { return __fadd_rn(a, b); }
-__device__ __forceinline__ double add_rn(double a, double b) { return __dadd_rn(a, b); }
+// kernels (square in place, then accumulate), so they must not contract into
+// a fused multiply-add (the pragma removes the contraction permission from
+// these operations, which survives inlining).
+__device__ __forceinline__ float mul_rn(float a, float b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+__device__ __forceinline__ double mul_rn(double a, double b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+__device__ __forceinline__ float add_rn(float a, float b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+__device__ __forceinline__ double add_rn(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
 
 // one tap, skipped when the weight is zero (as the reference skips it)
 template <typename F>
@@ -78,9 +92,7 @@ __global__ __launch_bounds__(256) void k_rgb2gray(const void* __restrict__ in, i
     } else {
       rd = (double)r; gd = (double)g; bd = (double)b;
     }
-    double y = __dmul_rn(0.2125, rd);
-    y = __dadd_rn(y, __dmul_rn(0.7154, gd));
-    y = __dadd_rn(y, __dmul_rn(0.0721, bd));
+    double y = add_rn(add_rn(mul_rn(0.2125, rd), mul_rn(0.7154, gd)), mul_rn(0.0721, bd));
     if (out_dt == HC_FLOAT32) ((float*)out)[i] = (float)y;
     else ((double*)out)[i] = y;
   }

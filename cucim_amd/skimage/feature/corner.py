@@ -86,7 +86,8 @@ def _hessian_matrix_with_gaussian(image, sigma=1, mode="reflect", cval=0, order=
     if order not in ["rc", "xy"]:
         raise ValueError(f"unrecognized order: {order}")
     ndim = image.dim()
-    if ndim in (2, 3) and image.numel() > 0:
+    commuting = not (mode in ("constant", "grid-constant") and float(cval) != 0.0)
+    if ndim in (2, 3) and image.numel() > 0 and commuting:
         x = image.contiguous()
         nH = ndim * (ndim + 1) // 2
         H = [_array.empty(x.shape, float_dtype) for _ in range(nH)]

@@ -90,6 +90,11 @@ def frangi(image, sigmas=range(1, 10, 2), scale_range=None, scale_step=None, alp
     out = _array.empty(image.shape, float_dtype)
     if _array.np_dtype(image.dtype) == np.float16:
         image = astype(image, np.float32)
+    if mode in ("constant", "grid-constant") and float(cval) != 0.0:
+        # the per-axis schedule reorders passes; with nonzero constant
+        # padding only the reference order is exact
+        return _frangi_sequential(image, sigmas, alpha, beta, gamma, black_ridges, mode,
+                                  cval, out)
     plan = FrangiPlan(image.shape, float_dtype, sigmas, mode)
     rc = plan.run(image, out, alpha, beta, gamma, black_ridges, cval)
     if rc == _lib.OK:

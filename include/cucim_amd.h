@@ -127,7 +127,10 @@ int hc_separable2d(const hc_array* in, const hc_array* out, const hc_taps* taps0
  * taps_s[d] / taps_d[d]: order-0 / order-1 kernels of axis d.  H: the
  * ndim*(ndim+1)/2 outputs (C-contiguous, image dtype) in
  * combinations_with_replacement order; order_xy != 0 reverses the axes
- * (2-D only).  No work memory is needed. */
+ * (2-D only).  No work memory is needed.  The passes run per axis, which
+ * reorders the reference's passes: exact for every mode except 'constant'
+ * with cval != 0, which returns HC_ERR_UNSUPPORTED (run the 1-D passes in
+ * the reference order with hc_correlate1d instead). */
 int hc_hessian_gaussian(const hc_array* in, const hc_array* H, int nH,
                         const hc_taps* taps_s, const hc_taps* taps_d, int mode,
                         double cval, int order_xy, void* stream);
@@ -142,7 +145,9 @@ int hc_sym_eigvals(const hc_array* S, int nS, const hc_array* eig, int sort,
  * nsig*ndim kernels (sigma-major).  gamma < 0 means "None" (gamma from the
  * first sigma, computed on the device; no host synchronisation).  work: 6
  * (3-D) or 3 (2-D) arrays of the image's shape in the work dtype;
- * scratch: >= 64 bytes of device memory.  out: image shape, work dtype. */
+ * scratch: >= 64 bytes of device memory.  out: image shape, work dtype.
+ * mode='constant' with cval != 0 returns HC_ERR_UNSUPPORTED (see
+ * hc_hessian_gaussian). */
 int hc_frangi(const hc_array* in, const hc_array* out, const hc_taps* taps_s,
               const hc_taps* taps_d, int nsig, double alpha, double beta,
               double gamma, int black_ridges, int mode, double cval,

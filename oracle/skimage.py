@@ -152,7 +152,9 @@ def _generic_edge_filter(image, *, smooth_weights, edge_weights=(1, 0, -1),
     output = None
     for i, edge_dim in enumerate(axes):
         ax = ndi.convolve1d(image, edge, axis=edge_dim, mode=mode, output=float_dtype)
-        for sd in sorted(set(range(ndim)) - {edge_dim % ndim}):
+        # edges.py:229 subtracts the axis as given: a negative axis is not
+        # removed, so the edge axis is smoothed as well
+        for sd in sorted(set(range(ndim)) - {edge_dim}):
             ax = ndi.convolve1d(ax, smooth, axis=sd, mode=mode, output=float_dtype)
         if return_magnitude:
             ax = ax * ax

@@ -1,10 +1,11 @@
 """GPU parity of the skimage-level functions against the oracle.
 
-Tolerances (float32 data): gaussian bit-exact against the float32-FMA
-emulation; Hessian elements within 2e-6 * max|H| of the FP64 oracle;
-eigenvalues within 1e-5 * max|lambda| of the oracle's closed form on the
-same float32 input; Frangi within max-abs 2e-4 and rel-L2 1e-4 of the FP64
-oracle.  rgb2gray bit-exact (all 2^24 uint8 triples).
+Tolerances (float32 data): gaussian within 2e-6 relative of the oracle;
+Hessian elements within 2e-6 * max|H| of the FP64 oracle; eigenvalues
+within 1e-5 * max|lambda| of the oracle's closed form on the same float32
+input; Frangi within max-abs 2e-4 and rel-L2 1e-4 of the FP64 oracle
+(constant mode with nonzero cval: 1e-3 / 3e-4).  rgb2gray bit-exact (all
+2^24 uint8 triples).  float64 data: 1e-9 relative or tighter.
 """
 
 import math
@@ -237,22 +238,29 @@ def _tubes(n, seed, ntubes=6):
 
 
 @pytest.mark.parametrize("n", [24, 48])
-@pytest.mark.parametrize("mode", ["reflect", "constant", "nearest"])
+@pytest.mark.parametrize("mode", ["reflect", "constant", "nearest", ("constant", 0.3)])
 @pytest.mark.parametrize("gamma", [None, 0.7])
 def test_frangi3d_vs_fp64(gpu, n, mode, gamma):
     from cucim_amd.skimage.filters import frangi
 
+    cval = 0.0
+    if isinstance(mode, tuple):
+        mode, cval = mode
     x = _tubes(n, seed=n)
     sig = (1, 2, 3)
-    got = _np(frangi(_t(x), sigmas=sig, black_ridges=False, mode=mode, gamma=gamma))
+    got = _np(frangi(_t(x), sigmas=sig, black_ridges=False, mode=mode, gamma=gamma, cval=cval))
     import scipy.ndimage as sndi
 
     exp = osk.frangi(x.astype(np.float64), sigmas=sig, black_ridges=False, mode=mode,
-                     gamma=gamma, ndi=sndi)
+                     gamma=gamma, ndi=sndi, cval=cval)
     assert got.dtype == np.float32
     err = np.abs(got - exp)
     rel_l2 = np.linalg.norm(got - exp) / max(np.linalg.norm(exp), 1e-30)
-    assert err.max() <= 2e-4 and rel_l2 <= 1e-4, (err.max(), rel_l2)
+    # nonzero constant padding makes a jump at the border that the Hessian
+    # amplifies: a float32 run of the algorithm is ~4e-5 rel-L2 from FP64
+    # there, ~1e-6 in the other modes
+    tol_max, tol_rel = (1e-3, 3e-4) if cval != 0 else (2e-4, 1e-4)
+    assert err.max() <= tol_max and rel_l2 <= tol_rel, (err.max(), rel_l2)
 
 
 @pytest.mark.parametrize("shape", [(64, 80), (300, 257)])
@@ -298,8 +306,15 @@ def test_frangi_invariants(gpu):
     a = _np(frangi(_t(x), sigmas=(1, 2), black_ridges=False))
     b = _np(frangi(_t(1.0 - x), sigmas=(1, 2), black_ridges=True))
     np.testing.assert_allclose(a, b, atol=1e-10, rtol=1e-10)
+    # test_ridges.py linearity inputs (uint8 ones, 1x vs 10x)
+    for shape in [(3, 3), (3, 3, 5)]:
+        o = np.ones(shape, np.uint8)
+        np.testing.assert_allclose(_np(frangi(_t(1 * o), black_ridges=True)),
+                                   _np(frangi(_t(10 * o), black_ridges=True)), atol=1e-3)
+    # the 1e-10 eigenvalue clamp is not scale-free: compare 10x against the oracle
     c = _np(frangi(_t(10 * x), sigmas=(1, 2), black_ridges=False))
-    np.testing.assert_allclose(a, c, atol=1e-3)
+    np.testing.assert_allclose(c, osk.frangi(10 * x, sigmas=(1, 2), black_ridges=False),
+                               rtol=1e-9, atol=1e-12)
 
 
 def test_frangi_fused_vs_reference_order(gpu):
