@@ -39,6 +39,8 @@ def _parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-edge", type=int, default=96,
                    help="edge of the cubic crop timed on the CPU")
+    p.add_argument("--only", choices=["all", "frangi", "correlate"], default="all",
+                   help="restrict the run (profiler passes)")
     return p.parse_args()
 
 
@@ -201,6 +203,9 @@ def main():
     import torch
 
     torch.cuda.set_device(0)
+    if args.only == "correlate":
+        print(json.dumps({"correlate1d": bench_correlate()}))
+        return
     vol, plan, ms, stats, rel_fused, max_fused = bench_frangi(args.steps, args.warmup)
     nvox = math.prod(SHAPE)
     value = nvox / (ms * 1e-3)
@@ -219,9 +224,9 @@ def main():
     else:
         fl = 0.0
     achieved = fl / (avg_ms * 1e-3) / 1e12 if fl else None
-    corr = bench_correlate()
+    corr = bench_correlate() if args.only == "all" else None
     cpu, parity = (None, None)
-    if not args.no_cpu:
+    if not args.no_cpu and args.only == "all":
         cpu, parity = cpu_baseline(vol, args.cpu_edge)
     line = {
         "metric": "Frangi voxels/s plain vs surrogate at rel-L2<=2%; correlate1d HBM GB/s",

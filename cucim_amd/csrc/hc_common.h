@@ -20,13 +20,18 @@ int hip_check(hipError_t e, const char* what);
 // a pair of hipEvents on its stream and tagged with the kernel's name;
 // hc_timing_collect() returns the elapsed times.
 int timing_begin(const char* name, hipStream_t st);
+int launch_error(hipError_t e, const char* name, size_t smem);
 void timing_end(int slot, hipStream_t st);
 
+// Launch with a clean error state, so a failure is attributed to this launch.
 #define HC_LAUNCH(name, st, kernel, grid, block, smem, ...)              \
   do {                                                                   \
+    (void)hipGetLastError();                                             \
     const int _slot = hc::timing_begin(name, st);                        \
     hipLaunchKernelGGL(kernel, grid, block, smem, st, __VA_ARGS__);      \
+    const hipError_t _le = hipGetLastError();                            \
     hc::timing_end(_slot, st);                                           \
+    if (_le != hipSuccess) return hc::launch_error(_le, name, smem);     \
   } while (0)
 
 #define HC_HIP(call)                                   \
@@ -83,14 +88,21 @@ __device__ __forceinline__ constexpr int w_dtype() {
 // LDS (loads are issued as a batch before their LDS stores).
 constexpr int kStageBatch = 8;
 
+template <typename W>
+__device__ __forceinline__ bool finite(W v) {
+  return v - v == (W)0;  // false for inf and NaN
+}
+
 // Stage `total` elements of a tile into LDS with kStageBatch global loads in
 // flight per lane.  map(e, slot) returns the global element index of tile
 // element e (or -1 for the constant-mode value) and sets its LDS slot (or -1
-// to skip it).  TYPED: the buffer holds W already.
+// to skip it).  TYPED: the buffer holds W already.  Returns whether this lane
+// staged a non-finite value.
 template <typename W, bool TYPED, class Map>
-__device__ __forceinline__ void stage_lds(W* __restrict__ lds, int total, int tid, int nthreads,
+__device__ __forceinline__ bool stage_lds(W* __restrict__ lds, int total, int tid, int nthreads,
                                           const void* __restrict__ in, int dt, W cval,
                                           bool negate, Map map) {
+  bool bad = false;
   for (int e0 = tid; e0 < total; e0 += nthreads * kStageBatch) {
     W v[kStageBatch];
     int slot[kStageBatch];
@@ -109,18 +121,21 @@ __device__ __forceinline__ void stage_lds(W* __restrict__ lds, int total, int ti
     }
 #pragma unroll
     for (int u = 0; u < kStageBatch; ++u)
-      if (slot[u] >= 0) lds[slot[u]] = v[u];
+      if (slot[u] >= 0) {
+        lds[slot[u]] = v[u];
+        bad |= !finite(v[u]);
+      }
   }
+  return bad;
 }
 
 template <typename W, class Map>
-__device__ __forceinline__ void stage_any(W* __restrict__ lds, int total, int tid, int nthreads,
+__device__ __forceinline__ bool stage_any(W* __restrict__ lds, int total, int tid, int nthreads,
                                           const void* __restrict__ in, int dt, W cval,
                                           bool negate, Map map) {
   if (dt == w_dtype<W>())
-    stage_lds<W, true>(lds, total, tid, nthreads, in, dt, cval, negate, map);
-  else
-    stage_lds<W, false>(lds, total, tid, nthreads, in, dt, cval, negate, map);
+    return stage_lds<W, true>(lds, total, tid, nthreads, in, dt, cval, negate, map);
+  return stage_lds<W, false>(lds, total, tid, nthreads, in, dt, cval, negate, map);
 }
 
 // Saturating truncation toward zero (float -> integer conversion of the GPU;
@@ -235,6 +250,19 @@ __device__ __host__ __forceinline__ int64_t remap(int64_t p, int64_t n, int mode
     default:
       return -1;
   }
+}
+
+// Work-groups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  Remap block indices so that each XCD takes a contiguous range of
+// tiles: neighbouring tiles, which share cache lines and halo rows, then sit
+// behind the same L2.
+__device__ __forceinline__ int64_t xcd_swizzle(int64_t b, int64_t nb) {
+  const int64_t nx = 8;
+  if (nb < 2 * nx) return b;
+  const int64_t per = nb / nx;
+  const int64_t full = per * nx;
+  if (b >= full) return b;
+  return (b % nx) * per + b / nx;
 }
 
 // ------------------------------------------------------------- weights --

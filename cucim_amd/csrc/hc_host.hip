@@ -19,6 +19,12 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+int launch_error(hipError_t e, const char* name, size_t smem) {
+  g_last_error = std::string("launch of ") + name + " failed (" + std::to_string(smem) +
+                 " bytes LDS): " + hipGetErrorString(e);
+  return HC_ERR_HIP;
+}
+
 int hip_check(hipError_t e, const char* what) {
   g_last_error = std::string("HIP error in ") + what + ": " + hipGetErrorString(e);
   return HC_ERR_HIP;

@@ -115,17 +115,6 @@ constexpr int kStrCols = 64;     // strided: lines per work-group
 constexpr int kStrStride = 65;   // strided: LDS row stride (elements)
 constexpr int kStrTA = 64;       // strided: outputs per line per work-group
 
-// Spread consecutive work-groups of a launch over the 8 XCDs in contiguous
-// chunks, so tiles that share halo rows sit behind the same L2.
-__device__ __forceinline__ int64_t xcd_swizzle(int64_t b, int64_t nb) {
-  const int64_t nx = 8;
-  if (nb < 2 * nx) return b;
-  const int64_t per = nb / nx;        // full rounds
-  const int64_t full = per * nx;
-  if (b >= full) return b;
-  return (b % nx) * per + b / nx;
-}
-
 template <typename W>
 __global__ __launch_bounds__(256) void k_tiled_contig(const TiledArgs<W> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -133,10 +122,13 @@ __global__ __launch_bounds__(256) void k_tiled_contig(const TiledArgs<W> a) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int L = kSeg + a.tp.K + 4;  // conv_row4 reads 8 past the last group
   W* ext = smem + wv * L;
+  W* wl = smem + 4 * L;
+  stage_taps<W>(wl, a.tp, threadIdx.x, 256);
   const int64_t nb = gridDim.x;
   const int64_t seg = xcd_swizzle(blockIdx.x, nb) * 4 + wv;
   const bool active = seg < a.nlines * a.nseg;
   int64_t lbase = 0, s0 = 0;
+  bool bad = false;
   if (active) {
     const int64_t line = seg / a.nseg;
     s0 = (seg - line * a.nseg) * kSeg;
@@ -144,26 +136,29 @@ __global__ __launch_bounds__(256) void k_tiled_contig(const TiledArgs<W> a) {
     const int64_t p0 = s0 - a.tp.anchor;
     const int64_t N = a.N, lb = lbase;
     const int mode = a.mode;
-    stage_any<W>(ext, L, lane, 64, a.in, a.in_dt, a.cval, false,
-                 [=](int e, int& slot) -> int64_t {
-                   slot = e;
-                   const int64_t r = remap(p0 + e, N, mode);
-                   return r < 0 ? -1 : lb + r;
-                 });
+    bad = stage_any<W>(ext, L, lane, 64, a.in, a.in_dt, a.cval, false,
+                       [=](int e, int& slot) -> int64_t {
+                         slot = e;
+                         const int64_t r = remap(p0 + e, N, mode);
+                         return r < 0 ? -1 : lb + r;
+                       });
   }
-  __syncthreads();
+  const bool skip = __syncthreads_or(bad);
   if (!active) return;
+  with_skip(skip, [&](auto sk) {
+    constexpr bool SK = decltype(sk)::value;
 #pragma unroll 1
-  for (int g = 0; g < kSeg / 256; ++g) {
-    const int o0 = 256 * g + 4 * lane;
-    const int64_t o = s0 + o0;
-    if (o >= a.N) break;
-    W acc[4] = {(W)0, (W)0, (W)0, (W)0};
-    conv_row4<W>(acc, ext + o0, a.tp);
+    for (int g = 0; g < kSeg / 256; ++g) {
+      const int o0 = 256 * g + 4 * lane;
+      const int64_t o = s0 + o0;
+      if (o >= a.N) break;
+      W acc[4] = {(W)0, (W)0, (W)0, (W)0};
+      conv_row4<SK>(acc, ext + o0, wl, a.tp);
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if (o + j < a.N) store_cast<W>(a.out, lbase + o + j, a.out_dt, acc[j]);
-  }
+      for (int j = 0; j < 4; ++j)
+        if (o + j < a.N) store_cast<W>(a.out, lbase + o + j, a.out_dt, acc[j]);
+    }
+  });
 }
 
 template <typename W>
@@ -179,31 +174,37 @@ __global__ __launch_bounds__(256) void k_tiled_strided(const TiledArgs<W> a) {
   const int64_t lo = lok ? L / a.inner : 0, li = lok ? L - lo * a.inner : 0;
   const int64_t lbase = lo * a.N * a.inner + li;
   const int64_t a0 = sg * kStrTA;
-  const int R = kStrTA + a.tp.K;
+  const int R = kStrTA + a.tp.K + 4;  // conv_col16 prefetches one group ahead
+  W* wl = ext + R * kStrStride;
+  stage_taps<W>(wl, a.tp, threadIdx.x, 256);
   const int64_t p0 = a0 - a.tp.anchor;
+  bool bad;
   {
     const int64_t N = a.N, inner = a.inner;
     const int mode = a.mode;
-    stage_any<W>(ext, R * 64, threadIdx.x, 256, a.in, a.in_dt, a.cval, false,
-                 [=](int e, int& slot) -> int64_t {
-                   const int q = e >> 6, c = e & 63;  // c == lane
-                   slot = q * kStrStride + c;
-                   if (!lok) return -1;
-                   const int64_t r = remap(p0 + q, N, mode);
-                   return r < 0 ? -1 : lbase + r * inner;
-                 });
+    bad = stage_any<W>(ext, R * 64, threadIdx.x, 256, a.in, a.in_dt, a.cval, false,
+                       [=](int e, int& slot) -> int64_t {
+                         const int q = e >> 6, c = e & 63;  // c == lane
+                         slot = q * kStrStride + c;
+                         if (!lok) return -1;
+                         const int64_t r = remap(p0 + q, N, mode);
+                         return r < 0 ? -1 : lbase + r * inner;
+                       });
   }
-  __syncthreads();
+  const bool skip = __syncthreads_or(bad);
   if (!lok) return;
   const int64_t n0 = a0 + 16 * wv;
   if (n0 >= a.N) return;
-  W acc[16];
+  with_skip(skip, [&](auto sk) {
+    constexpr bool SK = decltype(sk)::value;
+    W acc[16];
 #pragma unroll
-  for (int j = 0; j < 16; ++j) acc[j] = (W)0;
-  conv_col16<W, kStrStride>(acc, ext + (16 * wv) * kStrStride + lane, a.tp);
+    for (int j = 0; j < 16; ++j) acc[j] = (W)0;
+    conv_col16<SK, W, kStrStride>(acc, ext + (16 * wv) * kStrStride + lane, wl, a.tp);
 #pragma unroll
-  for (int j = 0; j < 16; ++j)
-    if (n0 + j < a.N) store_cast<W>(a.out, lbase + (n0 + j) * a.inner, a.out_dt, acc[j]);
+    for (int j = 0; j < 16; ++j)
+      if (n0 + j < a.N) store_cast<W>(a.out, lbase + (n0 + j) * a.inner, a.out_dt, acc[j]);
+  });
 }
 
 // ------------------------------------------------------ fused 2-D --
@@ -226,52 +227,67 @@ template <typename W, int TY>
 __global__ __launch_bounds__(256) void k_sep2d(const Sep2dArgs<W> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   W* A = reinterpret_cast<W*>(smem_raw);
-  const int RY = TY + a.t0.K;
+  const int RY = TY + a.t0.K + 4;  // conv_col16 prefetches one group ahead
   W* U = A + RY * kSepCols;
+  W* w0l = U + TY * kSepCols;
+  W* w1l = w0l + a.t0.K + 4;
+  stage_taps<W>(w0l, a.t0, threadIdx.x, 256);
+  stage_taps<W>(w1l, a.t1, threadIdx.x, 256);
   const int CX = a.TX + a.t1.K;
   const int nq = a.TX / 4;
   for (int64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     const int64_t ty = tile / a.ntx, tx = tile - ty * a.ntx;
     const int64_t y0 = ty * TY, x0 = tx * a.TX;
     // stage the input tile with both axes boundary-extended
+    bool bad;
     {
       const int64_t H = a.H, Wd = a.Wd, ya = y0 - a.t0.anchor, xa = x0 - a.t1.anchor;
       const int m0 = a.mode0, m1 = a.mode1;
-      stage_any<W>(A, RY * kSepCols, threadIdx.x, 256, a.in, a.in_dt, a.cval, false,
-                   [=](int e, int& slot) -> int64_t {
-                     const int q = e >> 7, c = e & (kSepCols - 1);
-                     slot = c < CX ? e : -1;
-                     const int64_t y = remap(ya + q, H, m0), x = remap(xa + c, Wd, m1);
-                     return (y < 0 || x < 0) ? -1 : y * Wd + x;
-                   });
+      bad = stage_any<W>(A, RY * kSepCols, threadIdx.x, 256, a.in, a.in_dt, a.cval, false,
+                         [=](int e, int& slot) -> int64_t {
+                           const int q = e >> 7, c = e & (kSepCols - 1);
+                           slot = c < CX ? e : -1;
+                           const int64_t y = remap(ya + q, H, m0), x = remap(xa + c, Wd, m1);
+                           return (y < 0 || x < 0) ? -1 : y * Wd + x;
+                         });
     }
-    __syncthreads();
+    const bool skip0 = __syncthreads_or(bad);
     // pass 1 (axis 0) over every staged column, rounded to the output dtype
-    for (int it = threadIdx.x; it < (TY / 16) * kSepCols; it += 256) {
-      const int ch = it / kSepCols, c = it - ch * kSepCols;
-      if (c >= CX) continue;
-      W acc[16];
+    bool bad1 = false;
+    with_skip(skip0, [&](auto sk) {
+      constexpr bool SK = decltype(sk)::value;
+      for (int it = threadIdx.x; it < (TY / 16) * kSepCols; it += 256) {
+        const int ch = it / kSepCols, c = it - ch * kSepCols;
+        if (c >= CX) continue;
+        W acc[16];
 #pragma unroll
-      for (int j = 0; j < 16; ++j) acc[j] = (W)0;
-      conv_col16<W, kSepCols>(acc, A + (16 * ch) * kSepCols + c, a.t0);
-      const int64_t xg = x0 - a.t1.anchor + c;
-      const bool xout = (xg < 0 || xg >= a.Wd) && a.mode1 == HC_MODE_CONSTANT;
+        for (int j = 0; j < 16; ++j) acc[j] = (W)0;
+        conv_col16<SK, W, kSepCols>(acc, A + (16 * ch) * kSepCols + c, w0l, a.t0);
+        const int64_t xg = x0 - a.t1.anchor + c;
+        const bool xout = (xg < 0 || xg >= a.Wd) && a.mode1 == HC_MODE_CONSTANT;
 #pragma unroll
-      for (int j = 0; j < 16; ++j)
-        U[(16 * ch + j) * kSepCols + c] = xout ? a.cval : roundtrip<W>(acc[j], a.out_dt);
-    }
-    __syncthreads();
+        for (int j = 0; j < 16; ++j) {
+          const W u = xout ? a.cval : roundtrip<W>(acc[j], a.out_dt);
+          bad1 |= !finite(u);
+          U[(16 * ch + j) * kSepCols + c] = u;
+        }
+      }
+    });
+    const bool skip1 = __syncthreads_or(bad1);
     // pass 2 (axis 1) along the rows of the intermediate
-    for (int it = threadIdx.x; it < TY * nq; it += 256) {
-      const int i = it / nq, o = (it - i * nq) * 4;
-      const int64_t y = y0 + i, x = x0 + o;
-      if (y >= a.H || x >= a.Wd) continue;
-      W acc[4] = {(W)0, (W)0, (W)0, (W)0};
-      conv_row4<W>(acc, U + i * kSepCols + o, a.t1);
+    with_skip(skip1, [&](auto sk) {
+      constexpr bool SK = decltype(sk)::value;
+      for (int it = threadIdx.x; it < TY * nq; it += 256) {
+        const int i = it / nq, o = (it - i * nq) * 4;
+        const int64_t y = y0 + i, x = x0 + o;
+        if (y >= a.H || x >= a.Wd) continue;
+        W acc[4] = {(W)0, (W)0, (W)0, (W)0};
+        conv_row4<SK>(acc, U + i * kSepCols + o, w1l, a.t1);
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (x + j < a.Wd) store_cast<W>(a.out, y * a.Wd + x + j, a.out_dt, acc[j]);
-    }
+        for (int j = 0; j < 4; ++j)
+          if (x + j < a.Wd) store_cast<W>(a.out, y * a.Wd + x + j, a.out_dt, acc[j]);
+      }
+    });
     __syncthreads();
   }
 }
@@ -352,7 +368,7 @@ int launch_tiled_1d(const hc_array* in, const hc_array* out, const hc_taps* taps
     const int64_t nsegs = a.nlines * a.nseg;
     const int64_t blocks = (nsegs + 3) / 4;
     if (blocks > 0x7fffffff) return fail(HC_ERR_VALUE, "array too large");
-    const size_t smem = (size_t)4 * (kSeg + a.tp.K + 4) * sizeof(W);
+    const size_t smem = ((size_t)4 * (kSeg + a.tp.K + 4) + a.tp.K + 4) * sizeof(W);
     HC_LAUNCH("k_tiled_contig", st, k_tiled_contig<W>, dim3((unsigned)blocks), dim3(256), smem, a);
   } else {
     a.nlines = ln.outer * ln.inner;
@@ -360,7 +376,8 @@ int launch_tiled_1d(const hc_array* in, const hc_array* out, const hc_taps* taps
     const int64_t nlb = (a.nlines + kStrCols - 1) / kStrCols;
     const int64_t blocks = nlb * a.nseg;
     if (blocks > 0x7fffffff) return fail(HC_ERR_VALUE, "array too large");
-    const size_t smem = (size_t)(kStrTA + a.tp.K) * kStrStride * sizeof(W);
+    const size_t smem =
+        ((size_t)(kStrTA + a.tp.K + 4) * kStrStride + a.tp.K + 4) * sizeof(W);
     HC_LAUNCH("k_tiled_strided", st, k_tiled_strided<W>, dim3((unsigned)blocks), dim3(256), smem, a);
   }
   HC_HIP(hipGetLastError());
@@ -404,11 +421,14 @@ int launch_gather_nd(const hc_array* in, const hc_array* out, const void* w_dev,
 
 template <typename W, int TY>
 int launch_sep2d_ty(const Sep2dArgs<W>& a, hipStream_t st, size_t smem) {
-  static bool attr_set = false;
-  if (!attr_set) {
-    (void)hipFuncSetAttribute((const void*)k_sep2d<W, TY>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr_set = true;
+  if (smem > 64 * 1024) {
+    const hipError_t e = hipFuncSetAttribute((const void*)k_sep2d<W, TY>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)smem);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(HC_ERR_RESOURCE, std::string("LDS request refused: ") + hipGetErrorString(e));
+    }
   }
   const int64_t grid = std::min<int64_t>(a.ntiles, 256 * 8);
   HC_LAUNCH("k_sep2d", st, (k_sep2d<W, TY>), dim3((unsigned)grid), dim3(256), smem, a);
@@ -442,7 +462,9 @@ int launch_sep2d(const hc_array* in, const hc_array* out, const hc_taps* t0, con
   a.ntx = (a.Wd + a.TX - 1) / a.TX;
   const int64_t nty = (a.H + TY - 1) / TY;
   a.ntiles = a.ntx * nty;
-  const size_t smem = (size_t)(TY + a.t0.K + TY) * kSepCols * sizeof(W);
+  const size_t smem =
+      ((size_t)(TY + a.t0.K + 4 + TY) * kSepCols + a.t0.K + a.t1.K + 8) * sizeof(W);
+  if (smem > 160 * 1024) return fail(HC_ERR_RESOURCE, "kernels too long for the fused 2-D tile");
   return launch_sep2d_ty<W, TY>(a, st, smem);
 }
 

@@ -21,6 +21,7 @@
 // before writing the same lines.
 #include <algorithm>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "hc_conv.h"
@@ -30,7 +31,7 @@ namespace hc {
 
 constexpr int kChC = 16;        // strided chain pass: lines per work-group
 constexpr int kChS = kChC + 1;  // LDS row stride
-constexpr int kFinIPT = 4;      // final pass: 4-output items per thread
+constexpr int kFinIPT = 2;      // final pass: 4-output items per thread
 constexpr int kMaxLds = 160 * 1024;
 
 struct ChainOut {
@@ -60,58 +61,101 @@ struct ChainArgs {
 };
 
 // --------------------------------------------------- strided chain pass --
-template <typename W>
+// Two LDS line buffers: the input lines are staged in A; the first passes
+// (smoothing and/or derivative) run into registers, then, after a barrier,
+// overwrite A (smoothing) and B (derivative); the second passes read A / B.
+constexpr int kChP = 8;  // outputs per lane along a line
+constexpr int kChMaxItems = 4;  // first-pass items a lane holds in registers
+
+template <typename W, int NI>
 __global__ __launch_bounds__(256) void k_chain_strided(const ChainArgs<W> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  W* xin = reinterpret_cast<W*>(smem_raw);
+  W* bufA = reinterpret_cast<W*>(smem_raw);
   const int E = a.E, Nr = a.Nr;
-  W* xs = xin + E * kChS;
-  W* xd = xs + E * kChS;
+  W* bufB = bufA + E * kChS;
+  W* wS = bufB + E * kChS;
+  W* wD = wS + a.S.K + 4;
   const ChainJob& jb = a.job[blockIdx.y];
   const int anc = a.S.anchor;
-  const int64_t L0 = (int64_t)blockIdx.x * kChC;
+  const int64_t L0 = xcd_swizzle(blockIdx.x, gridDim.x) * kChC;
   const int tid = threadIdx.x;
+  stage_taps<W>(wS, a.S, tid, 256);
+  stage_taps<W>(wD, a.D, tid, 256);
   bool need_s = false, need_d = false;
   for (int k = 0; k < jb.nout; ++k) {
     need_s |= jb.o[k].first == 0;
     need_d |= jb.o[k].first == 1;
   }
-  // 1. stage whole lines (+ halo) of the job's input
+  // 1. stage whole lines (+ halo) of the job's input into A
+  bool bad;
   {
     const int64_t N = a.N, inner = a.inner, nlines = a.nlines;
     const int mode = a.mode;
-    stage_any<W>(xin, E * kChC, tid, 256, jb.in, jb.in_dt, a.cval, a.negate != 0,
-                 [=](int e, int& slot) -> int64_t {
-                   const int q = e / kChC, c = e - q * kChC;
-                   slot = q * kChS + c;
-                   const int64_t L = L0 + c;
-                   if (L >= nlines) return -1;
-                   const int64_t r = remap((int64_t)q - anc, N, mode);
-                   if (r < 0) return -1;
-                   const int64_t lo = L / inner, li = L - lo * inner;
-                   return lo * N * inner + li + r * inner;
-                 });
+    bad = stage_any<W>(bufA, E * kChC, tid, 256, jb.in, jb.in_dt, a.cval, a.negate != 0,
+                       [=](int e, int& slot) -> int64_t {
+                         const int q = e / kChC, c = e - q * kChC;
+                         slot = q * kChS + c;
+                         const int64_t L = L0 + c;
+                         if (L >= nlines) return -1;
+                         const int64_t r = remap((int64_t)q - anc, N, mode);
+                         if (r < 0) return -1;
+                         const int64_t lo = L / inner, li = L - lo * inner;
+                         return lo * N * inner + li + r * inner;
+                       });
   }
-  __syncthreads();
-  // 2. first passes over positions [0, Nr)
-  const int nitems = (Nr / 16) * kChC;
-  for (int it = tid; it < nitems; it += 256) {
-    const int ch = it / kChC, c = it - ch * kChC;
-    const W* x = xin + (16 * ch) * kChS + c;
-    W as[16], ad[16];
+  const bool skip0 = __syncthreads_or(bad);
+  // 2. first passes over positions [0, Nr) into registers (<= NI items/lane)
+  const int nitems = (Nr / kChP) * kChC;
+  W fs[NI][kChP], fd[NI][kChP];
+  with_skip(skip0, [&](auto sk) {
+    constexpr bool SK = decltype(sk)::value;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) { as[j] = (W)0; ad[j] = (W)0; }
-    if (need_s && need_d) conv_col16x2<W, kChS>(as, ad, x, a.S, a.D);
-    else if (need_s) conv_col16<W, kChS>(as, x, a.S);
-    else conv_col16<W, kChS>(ad, x, a.D);
+    for (int i = 0; i < NI; ++i) {
+      const int it = tid + 256 * i;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const int q = anc + 16 * ch + j;
-      xs[q * kChS + c] = as[j];
-      xd[q * kChS + c] = ad[j];
+      for (int j = 0; j < kChP; ++j) { fs[i][j] = (W)0; fd[i][j] = (W)0; }
+      if (it < nitems) {
+        const int ch = it / kChC, c = it - ch * kChC;
+        const W* x = bufA + (kChP * ch) * kChS + c;
+        W acc[2][kChP];
+#pragma unroll
+        for (int j = 0; j < kChP; ++j) { acc[0][j] = (W)0; acc[1][j] = (W)0; }
+        if (need_s && need_d) {
+          const W* const wls[2] = {wS, wD};
+          const TapPack<W>* const tps[2] = {&a.S, &a.D};
+          conv_colP<W, kChS, 2, kChP, SK>(acc, x, wls, tps, a.S.K);
+        } else {
+          W acc1[1][kChP];
+#pragma unroll
+          for (int j = 0; j < kChP; ++j) acc1[0][j] = (W)0;
+          const W* const wls[1] = {need_s ? wS : wD};
+          const TapPack<W>* const tps[1] = {need_s ? &a.S : &a.D};
+          conv_colP<W, kChS, 1, kChP, SK>(acc1, x, wls, tps, a.S.K);
+#pragma unroll
+          for (int j = 0; j < kChP; ++j) acc[need_s ? 0 : 1][j] = acc1[0][j];
+        }
+#pragma unroll
+        for (int j = 0; j < kChP; ++j) { fs[i][j] = acc[0][j]; fd[i][j] = acc[1][j]; }
+      }
+    }
+  });
+  __syncthreads();  // every read of the staged input is done
+  bool bad1 = false;
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int it = tid + 256 * i;
+    if (it < nitems) {
+      const int ch = it / kChC, c = it - ch * kChC;
+#pragma unroll
+      for (int j = 0; j < kChP; ++j) {
+        const int q = anc + kChP * ch + j;
+        bufA[q * kChS + c] = fs[i][j];
+        bufB[q * kChS + c] = fd[i][j];
+        bad1 |= !(finite(fs[i][j]) && finite(fd[i][j]));
+      }
     }
   }
-  __syncthreads();
+  const bool skip1 = __syncthreads_or(bad1) || skip0;
   // 3. boundary-extend the first-pass results
   const int nh = E - a.N;  // positions [0, anc) and [anc + N, E)
   for (int e = tid; e < nh * kChC; e += 256) {
@@ -120,51 +164,66 @@ __global__ __launch_bounds__(256) void k_chain_strided(const ChainArgs<W> a) {
     const int64_t r = remap((int64_t)q - anc, a.N, a.mode);
     W vs = a.cval, vd = a.cval;
     if (r >= 0) {
-      vs = xs[(r + anc) * kChS + c];
-      vd = xd[(r + anc) * kChS + c];
+      vs = bufA[(r + anc) * kChS + c];
+      vd = bufB[(r + anc) * kChS + c];
     }
-    xs[q * kChS + c] = vs;
-    xd[q * kChS + c] = vd;
+    bufA[q * kChS + c] = vs;
+    bufB[q * kChS + c] = vd;
   }
   __syncthreads();
   // 4. second passes, sharing window reads between outputs of one source
-  for (int it = tid; it < nitems; it += 256) {
-    const int ch = it / kChC, c = it - ch * kChC;
-    const int64_t L = L0 + c;
-    if (L >= a.nlines) continue;
-    const int64_t lo = L / a.inner, li = L - lo * a.inner;
-    const int64_t base = lo * a.N * a.inner + li;
+  with_skip(skip1, [&](auto sk) {
+    constexpr bool SK = decltype(sk)::value;
+    for (int it = tid; it < nitems; it += 256) {
+      const int ch = it / kChC, c = it - ch * kChC;
+      const int64_t L = L0 + c;
+      if (L >= a.nlines) continue;
+      const int64_t lo = L / a.inner, li = L - lo * a.inner;
+      const int64_t base = lo * a.N * a.inner + li;
 #pragma unroll 1
-    for (int src = 0; src < 2; ++src) {
-      int ks = -1, kd = -1;
-      for (int k = 0; k < jb.nout; ++k) {
-        if (jb.o[k].first != src) continue;
-        if (jb.o[k].second == 0) ks = k; else kd = k;
-      }
-      if (ks < 0 && kd < 0) continue;
-      const W* x = (src ? xd : xs) + (16 * ch) * kChS + c;
-      W r0[16], r1[16];
+      for (int src = 0; src < 2; ++src) {
+        int ks = -1, kd = -1;
+        for (int k = 0; k < jb.nout; ++k) {
+          if (jb.o[k].first != src) continue;
+          if (jb.o[k].second == 0) ks = k; else kd = k;
+        }
+        if (ks < 0 && kd < 0) continue;
+        const W* x = (src ? bufB : bufA) + (kChP * ch) * kChS + c;
+        W r[2][kChP];
 #pragma unroll
-      for (int j = 0; j < 16; ++j) { r0[j] = (W)0; r1[j] = (W)0; }
-      if (ks >= 0 && kd >= 0) conv_col16x2<W, kChS>(r0, r1, x, a.S, a.D);
-      else if (ks >= 0) conv_col16<W, kChS>(r0, x, a.S);
-      else conv_col16<W, kChS>(r1, x, a.D);
+        for (int j = 0; j < kChP; ++j) { r[0][j] = (W)0; r[1][j] = (W)0; }
+        if (ks >= 0 && kd >= 0) {
+          const W* const wls[2] = {wS, wD};
+          const TapPack<W>* const tps[2] = {&a.S, &a.D};
+          conv_colP<W, kChS, 2, kChP, SK>(r, x, wls, tps, a.S.K);
+        } else {
+          W r1[1][kChP];
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int64_t n = 16 * ch + j;
-        if (n < a.N) {
-          if (ks >= 0) reinterpret_cast<W*>(jb.o[ks].out)[base + n * a.inner] = r0[j];
-          if (kd >= 0) reinterpret_cast<W*>(jb.o[kd].out)[base + n * a.inner] = r1[j];
+          for (int j = 0; j < kChP; ++j) r1[0][j] = (W)0;
+          const W* const wls[1] = {ks >= 0 ? wS : wD};
+          const TapPack<W>* const tps[1] = {ks >= 0 ? &a.S : &a.D};
+          conv_colP<W, kChS, 1, kChP, SK>(r1, x, wls, tps, a.S.K);
+#pragma unroll
+          for (int j = 0; j < kChP; ++j) r[ks >= 0 ? 0 : 1][j] = r1[0][j];
+        }
+        W* os = ks >= 0 ? reinterpret_cast<W*>(jb.o[ks].out) : nullptr;
+        W* od = kd >= 0 ? reinterpret_cast<W*>(jb.o[kd].out) : nullptr;
+#pragma unroll
+        for (int j = 0; j < kChP; ++j) {
+          const int64_t n = kChP * ch + j;
+          if (n < a.N) {
+            if (os) os[base + n * a.inner] = r[0][j];
+            if (od) od[base + n * a.inner] = r[1][j];
+          }
         }
       }
     }
-  }
+  });
 }
 
 // ------------------------------------------------ contiguous chain pass --
-// Only used when the first Hessian pass runs along a contiguous axis (1-D
-// images are not on this path; kept for 2-D arrays whose axis-0 lines are
-// too long for the strided pass).  Lines are rows.
+// The first Hessian pass when it runs along a contiguous axis (2-D arrays
+// whose axis-0 lines are too long for the strided pass).  Lines are rows.
 template <typename W>
 __global__ __launch_bounds__(256) void k_chain_contig(const ChainArgs<W> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -172,44 +231,54 @@ __global__ __launch_bounds__(256) void k_chain_contig(const ChainArgs<W> a) {
   W* xin = reinterpret_cast<W*>(smem_raw);
   W* xs = xin + R * E;
   W* xd = xs + R * E;
+  W* wS = xd + R * E;
+  W* wD = wS + a.S.K + 4;
   const ChainJob& jb = a.job[blockIdx.y];
   const int anc = a.S.anchor;
   const int64_t row0 = (int64_t)blockIdx.x * R;
   const int tid = threadIdx.x;
+  stage_taps<W>(wS, a.S, tid, 256);
+  stage_taps<W>(wD, a.D, tid, 256);
   bool need_s = false, need_d = false;
   for (int k = 0; k < jb.nout; ++k) {
     need_s |= jb.o[k].first == 0;
     need_d |= jb.o[k].first == 1;
   }
+  bool bad;
   {
     const int64_t N = a.N, nlines = a.nlines;
     const int mode = a.mode;
-    stage_any<W>(xin, R * E, tid, 256, jb.in, jb.in_dt, a.cval, a.negate != 0,
-                 [=](int e, int& slot) -> int64_t {
-                   const int rr = e / E, q = e - rr * E;
-                   slot = e;
-                   const int64_t row = row0 + rr;
-                   if (row >= nlines) return -1;
-                   const int64_t r = remap((int64_t)q - anc, N, mode);
-                   return r < 0 ? -1 : row * N + r;
-                 });
+    bad = stage_any<W>(xin, R * E, tid, 256, jb.in, jb.in_dt, a.cval, a.negate != 0,
+                       [=](int e, int& slot) -> int64_t {
+                         const int rr = e / E, q = e - rr * E;
+                         slot = e;
+                         const int64_t row = row0 + rr;
+                         if (row >= nlines) return -1;
+                         const int64_t r = remap((int64_t)q - anc, N, mode);
+                         return r < 0 ? -1 : row * N + r;
+                       });
   }
-  __syncthreads();
+  const bool skip0 = __syncthreads_or(bad);
   const int nq = Nr / 4;
-  for (int it = tid; it < R * nq; it += 256) {
-    const int rr = it / nq, o = (it - rr * nq) * 4;
-    W as[4] = {(W)0, (W)0, (W)0, (W)0}, ad[4] = {(W)0, (W)0, (W)0, (W)0};
-    const W* x = xin + rr * E + o;
-    if (need_s && need_d) conv_row4x2<W>(as, ad, x, a.S, a.D);
-    else if (need_s) conv_row4<W>(as, x, a.S);
-    else conv_row4<W>(ad, x, a.D);
+  bool bad1 = false;
+  with_skip(skip0, [&](auto sk) {
+    constexpr bool SK = decltype(sk)::value;
+    for (int it = tid; it < R * nq; it += 256) {
+      const int rr = it / nq, o = (it - rr * nq) * 4;
+      W as[4] = {(W)0, (W)0, (W)0, (W)0}, ad[4] = {(W)0, (W)0, (W)0, (W)0};
+      const W* x = xin + rr * E + o;
+      if (need_s && need_d) conv_row4x2<SK>(as, ad, x, wS, a.S, wD, a.D);
+      else if (need_s) conv_row4<SK>(as, x, wS, a.S);
+      else conv_row4<SK>(ad, x, wD, a.D);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      xs[rr * E + anc + o + j] = as[j];
-      xd[rr * E + anc + o + j] = ad[j];
+      for (int j = 0; j < 4; ++j) {
+        xs[rr * E + anc + o + j] = as[j];
+        xd[rr * E + anc + o + j] = ad[j];
+        bad1 |= !(finite(as[j]) && finite(ad[j]));
+      }
     }
-  }
-  __syncthreads();
+  });
+  const bool skip1 = __syncthreads_or(bad1) || skip0;
   const int nh = E - (int)a.N;
   for (int e = tid; e < R * nh; e += 256) {
     const int rr = e / nh, h = e - rr * nh;
@@ -224,32 +293,35 @@ __global__ __launch_bounds__(256) void k_chain_contig(const ChainArgs<W> a) {
     xd[rr * E + q] = vd;
   }
   __syncthreads();
-  for (int it = tid; it < R * nq; it += 256) {
-    const int rr = it / nq, o = (it - rr * nq) * 4;
-    const int64_t row = row0 + rr;
-    if (row >= a.nlines) continue;
+  with_skip(skip1, [&](auto sk) {
+    constexpr bool SK = decltype(sk)::value;
+    for (int it = tid; it < R * nq; it += 256) {
+      const int rr = it / nq, o = (it - rr * nq) * 4;
+      const int64_t row = row0 + rr;
+      if (row >= a.nlines) continue;
 #pragma unroll 1
-    for (int src = 0; src < 2; ++src) {
-      int ks = -1, kd = -1;
-      for (int k = 0; k < jb.nout; ++k) {
-        if (jb.o[k].first != src) continue;
-        if (jb.o[k].second == 0) ks = k; else kd = k;
-      }
-      if (ks < 0 && kd < 0) continue;
-      const W* x = (src ? xd : xs) + rr * E + o;
-      W r0[4] = {(W)0, (W)0, (W)0, (W)0}, r1[4] = {(W)0, (W)0, (W)0, (W)0};
-      if (ks >= 0 && kd >= 0) conv_row4x2<W>(r0, r1, x, a.S, a.D);
-      else if (ks >= 0) conv_row4<W>(r0, x, a.S);
-      else conv_row4<W>(r1, x, a.D);
+      for (int src = 0; src < 2; ++src) {
+        int ks = -1, kd = -1;
+        for (int k = 0; k < jb.nout; ++k) {
+          if (jb.o[k].first != src) continue;
+          if (jb.o[k].second == 0) ks = k; else kd = k;
+        }
+        if (ks < 0 && kd < 0) continue;
+        const W* x = (src ? xd : xs) + rr * E + o;
+        W r0[4] = {(W)0, (W)0, (W)0, (W)0}, r1[4] = {(W)0, (W)0, (W)0, (W)0};
+        if (ks >= 0 && kd >= 0) conv_row4x2<SK>(r0, r1, x, wS, a.S, wD, a.D);
+        else if (ks >= 0) conv_row4<SK>(r0, x, wS, a.S);
+        else conv_row4<SK>(r1, x, wD, a.D);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if (o + j < a.N) {
-          if (ks >= 0) reinterpret_cast<W*>(jb.o[ks].out)[row * a.N + o + j] = r0[j];
-          if (kd >= 0) reinterpret_cast<W*>(jb.o[kd].out)[row * a.N + o + j] = r1[j];
+        for (int j = 0; j < 4; ++j) {
+          if (o + j < a.N) {
+            if (ks >= 0) reinterpret_cast<W*>(jb.o[ks].out)[row * a.N + o + j] = r0[j];
+            if (kd >= 0) reinterpret_cast<W*>(jb.o[kd].out)[row * a.N + o + j] = r1[j];
+          }
         }
       }
     }
-  }
+  });
 }
 
 // ----------------------------------------------------------- final pass --
@@ -300,74 +372,104 @@ __device__ __forceinline__ void atomic_max_bits<double>(void* p, double v) {
   if ((threadIdx.x & 63) == 0) atomicMax(reinterpret_cast<unsigned long long*>(p), b);
 }
 
-template <typename W, int NIN>
-__global__ __launch_bounds__(256) void k_final_contig(const FinalArgs<W> a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int E = a.E, Nr = a.Nr, R = a.rows;
-  W* xin = reinterpret_cast<W*>(smem_raw);
-  W* xf = xin + R * E;
-  const int anc = a.S.anchor;
-  const int64_t row0 = (int64_t)blockIdx.x * R;
-  const int tid = threadIdx.x;
-  const int nq = Nr / 4;
-  const int nitems = R * nq;
-  W res[NIN][kFinIPT][4];
+// One input of the final pass: stage its rows, first op, re-extend, second
+// op into res (IPT items of 4 outputs per thread).
+template <typename W, int IPT, int J>
+__device__ __forceinline__ void final_input(const FinalArgs<W>& a, W* xin, W* xf, const W* wS,
+                                            const W* wD, int64_t row0, W (&res)[IPT][4]) {
+  const int E = a.E, R = a.rows, anc = a.S.anchor, tid = threadIdx.x;
+  const int nq = a.Nr / 4, nitems = R * nq;
+  bool bad;
+  {
+    const int64_t N = a.N, nrows = a.nrows;
+    const int mode = a.mode;
+    bad = stage_lds<W, true>(xin, R * E, tid, 256, a.in[J], w_dtype<W>(), a.cval, false,
+                             [=](int e, int& slot) -> int64_t {
+                               const int rr = e / E, q = e - rr * E;
+                               slot = e;
+                               const int64_t row = row0 + rr;
+                               if (row >= nrows) return -1;
+                               const int64_t r = remap((int64_t)q - anc, N, mode);
+                               return r < 0 ? -1 : row * N + r;
+                             });
+  }
+  const bool skip0 = __syncthreads_or(bad);
+  const TapPack<W>& t1 = a.first[J] ? a.D : a.S;
+  const W* w1 = a.first[J] ? wD : wS;
+  bool bad1 = false;
+  with_skip(skip0, [&](auto sk) {
+    constexpr bool SK = decltype(sk)::value;
 #pragma unroll
-  for (int j = 0; j < NIN; ++j) {
-    {
-      const int64_t N = a.N, nrows = a.nrows;
-      const int mode = a.mode;
-      stage_lds<W, true>(xin, R * E, tid, 256, a.in[j], w_dtype<W>(), a.cval, false,
-                         [=](int e, int& slot) -> int64_t {
-                           const int rr = e / E, q = e - rr * E;
-                           slot = e;
-                           const int64_t row = row0 + rr;
-                           if (row >= nrows) return -1;
-                           const int64_t r = remap((int64_t)q - anc, N, mode);
-                           return r < 0 ? -1 : row * N + r;
-                         });
-    }
-    __syncthreads();
-    const TapPack<W>& t1 = a.first[j] ? a.D : a.S;
-#pragma unroll
-    for (int i = 0; i < kFinIPT; ++i) {
+    for (int i = 0; i < IPT; ++i) {
       const int it = tid + 256 * i;
       if (it < nitems) {
         const int rr = it / nq, o = (it - rr * nq) * 4;
         W acc[4] = {(W)0, (W)0, (W)0, (W)0};
-        conv_row4<W>(acc, xin + rr * E + o, t1);
+        conv_row4<SK>(acc, xin + rr * E + o, w1, t1);
 #pragma unroll
-        for (int m = 0; m < 4; ++m) xf[rr * E + anc + o + m] = acc[m];
+        for (int m = 0; m < 4; ++m) {
+          xf[rr * E + anc + o + m] = acc[m];
+          bad1 |= !finite(acc[m]);
+        }
       }
     }
-    __syncthreads();
-    const int nh = E - (int)a.N;
-    for (int e = tid; e < R * nh; e += 256) {
-      const int rr = e / nh, h = e - rr * nh;
-      const int q = h < anc ? h : h + (int)a.N;
-      const int64_t r = remap((int64_t)q - anc, a.N, a.mode);
-      xf[rr * E + q] = r >= 0 ? xf[rr * E + r + anc] : a.cval;
-    }
-    __syncthreads();
-    const TapPack<W>& t2 = a.second[j] ? a.D : a.S;
+  });
+  const bool skip1 = __syncthreads_or(bad1) || skip0;
+  const int nh = E - (int)a.N;
+  for (int e = tid; e < R * nh; e += 256) {
+    const int rr = e / nh, h = e - rr * nh;
+    const int q = h < anc ? h : h + (int)a.N;
+    const int64_t r = remap((int64_t)q - anc, a.N, a.mode);
+    xf[rr * E + q] = r >= 0 ? xf[rr * E + r + anc] : a.cval;
+  }
+  __syncthreads();
+  const TapPack<W>& t2 = a.second[J] ? a.D : a.S;
+  const W* w2 = a.second[J] ? wD : wS;
+  with_skip(skip1, [&](auto sk) {
+    constexpr bool SK = decltype(sk)::value;
 #pragma unroll
-    for (int i = 0; i < kFinIPT; ++i) {
+    for (int i = 0; i < IPT; ++i) {
       const int it = tid + 256 * i;
 #pragma unroll
-      for (int m = 0; m < 4; ++m) res[j][i][m] = (W)0;
+      for (int m = 0; m < 4; ++m) res[i][m] = (W)0;
       if (it < nitems) {
         const int rr = it / nq, o = (it - rr * nq) * 4;
-        conv_row4<W>(res[j][i], xf + rr * E + o, t2);
+        conv_row4<SK>(res[i], xf + rr * E + o, w2, t2);
       }
     }
-    __syncthreads();
-  }
+  });
+  __syncthreads();
+}
+
+template <typename W, int NIN, int IPT, size_t... Js>
+__device__ __forceinline__ void final_all(const FinalArgs<W>& a, W* xin, W* xf, const W* wS,
+                                          const W* wD, int64_t row0, W (&res)[NIN][IPT][4],
+                                          std::index_sequence<Js...>) {
+  (final_input<W, IPT, (int)Js>(a, xin, xf, wS, wD, row0, res[Js]), ...);
+}
+
+template <typename W, int NIN, int IPT>
+__global__ __launch_bounds__(256) void k_final_contig(const FinalArgs<W> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int E = a.E, R = a.rows;
+  W* xin = reinterpret_cast<W*>(smem_raw);
+  W* xf = xin + R * E;
+  W* wS = xf + R * E;
+  W* wD = wS + a.S.K + 4;
+  const int64_t row0 = xcd_swizzle(blockIdx.x, gridDim.x) * R;
+  const int tid = threadIdx.x;
+  const int nq = a.Nr / 4;
+  const int nitems = R * nq;
+  stage_taps<W>(wS, a.S, tid, 256);
+  stage_taps<W>(wD, a.D, tid, 256);
+  W res[NIN][IPT][4];
+  final_all<W, NIN, IPT>(a, xin, xf, wS, wD, row0, res, std::make_index_sequence<NIN>{});
   // epilogue: every item holds all Hessian elements of its 4 voxels
   double gsq = a.gamma_sq;
   if (a.epi == EPI_FRANGI && a.gamma_sq_dev) gsq = *a.gamma_sq_dev;
   W rgmax = (W)0;
 #pragma unroll
-  for (int i = 0; i < kFinIPT; ++i) {
+  for (int i = 0; i < IPT; ++i) {
     const int it = tid + 256 * i;
     if (it >= nitems) continue;
     const int rr = it / nq, o = (it - rr * nq) * 4;
@@ -385,8 +487,8 @@ __global__ __launch_bounds__(256) void k_final_contig(const FinalArgs<W> a) {
       if (NIN == 6) {
         W l1, l2, l3;
         // H order (H00, H01, H02, H11, H12, H22) = (a, d, f, b, e, c)
-        eig33<W>(res[0][i][m], res[3][i][m], res[5][i][m], res[1][i][m], res[4][i][m],
-                 res[2][i][m], 0, 1, l1, l2, l3);
+        eig33<W>(res[0][i][m], res[3 % NIN][i][m], res[5 % NIN][i][m], res[1][i][m],
+                 res[4 % NIN][i][m], res[2][i][m], 0, 1, l1, l2, l3);
         const W rg = frangi_rg3<W>(l1, l2, l3);
         if (a.store_eig) {
           a.eig[0][idx] = l1;
@@ -533,7 +635,7 @@ inline int chain_first(const Pair& p, int x) { return p.a == x ? 1 : 0; }
 inline int chain_second(const Pair& p, int x) { return p.b == x ? 1 : 0; }
 
 template <typename W>
-int lds_strided(int E) { return 3 * E * kChS * (int)sizeof(W); }
+int lds_strided(int E) { return 2 * E * kChS * (int)sizeof(W); }
 
 template <typename W>
 int pick_rows(int E, int Nr, int nbuf, int max_items) {
@@ -577,18 +679,26 @@ int run_chain_pass(const ChainJob* jobs, int njobs, const hc_array* shape_src, i
   a.D = tp.D;
   if (ln.inner > 1) {
     a.nlines = ln.outer * ln.inner;
-    a.Nr = (int)((ln.N + 15) / 16 * 16);
-    a.E = a.Nr + a.S.K;
-    const int lds = lds_strided<W>(a.E);
+    a.Nr = (int)((ln.N + kChP - 1) / kChP * kChP);
+    a.E = a.Nr + a.S.K + 4;  // the column core prefetches one group ahead
+    const int nitems = (a.Nr / kChP) * kChC;
+    if (nitems > 256 * kChMaxItems)
+      return fail(HC_ERR_RESOURCE, "line too long for the fused Hessian");
+    const int lds = lds_strided<W>(a.E) + 2 * (a.S.K + 4) * (int)sizeof(W);
     if (lds > kMaxLds) return fail(HC_ERR_RESOURCE, "line too long for the fused Hessian");
     const int64_t blocks = (a.nlines + kChC - 1) / kChC;
-    HC_LAUNCH("k_chain_strided", st, k_chain_strided<W>, dim3((unsigned)blocks, njobs), dim3(256), lds, a);
+    if (nitems <= 512)
+      HC_LAUNCH("k_chain_strided", st, (k_chain_strided<W, 2>), dim3((unsigned)blocks, njobs),
+                dim3(256), lds, a);
+    else
+      HC_LAUNCH("k_chain_strided", st, (k_chain_strided<W, kChMaxItems>),
+                dim3((unsigned)blocks, njobs), dim3(256), lds, a);
   } else {
     a.nlines = ln.outer;
     a.Nr = (int)((ln.N + 3) / 4 * 4);
     a.E = a.Nr + a.S.K + 4;
     a.rows = pick_rows<W>(a.E, a.Nr, 3, 1024);
-    const int lds = 3 * a.rows * a.E * (int)sizeof(W);
+    const int lds = (3 * a.rows * a.E + 2 * (a.S.K + 4)) * (int)sizeof(W);
     if (lds > kMaxLds) return fail(HC_ERR_RESOURCE, "line too long for the fused Hessian");
     const int64_t blocks = (a.nlines + a.rows - 1) / a.rows;
     HC_LAUNCH("k_chain_contig", st, k_chain_contig<W>, dim3((unsigned)blocks, njobs), dim3(256), lds, a);
@@ -620,12 +730,14 @@ int prepare_final(FinalArgs<W>* a, const hc_array* shape_src, const AxisTaps<W>&
 
 template <typename W>
 int launch_final(const FinalArgs<W>& a, hipStream_t st) {
-  const int lds = 2 * a.rows * a.E * (int)sizeof(W);
+  const int lds = (2 * a.rows * a.E + 2 * (a.S.K + 4)) * (int)sizeof(W);
   const int64_t blocks = (a.nrows + a.rows - 1) / a.rows;
   if (a.nin == 6)
-    HC_LAUNCH("k_final_contig", st, (k_final_contig<W, 6>), dim3((unsigned)blocks), dim3(256), lds, a);
+    HC_LAUNCH("k_final_contig", st, (k_final_contig<W, 6, kFinIPT>), dim3((unsigned)blocks),
+              dim3(256), lds, a);
   else
-    HC_LAUNCH("k_final_contig", st, (k_final_contig<W, 3>), dim3((unsigned)blocks), dim3(256), lds, a);
+    HC_LAUNCH("k_final_contig", st, (k_final_contig<W, 3, kFinIPT>), dim3((unsigned)blocks),
+              dim3(256), lds, a);
   HC_HIP(hipGetLastError());
   return HC_OK;
 }
@@ -636,8 +748,9 @@ template <typename W>
 int plan_fits(const hc_array* img, const AxisTaps<W>* tp) {
   const int nd = img->ndim;
   for (int x = 0; x < nd - 1; ++x) {
-    const int Nr = (int)((img->shape[x] + 15) / 16 * 16);
-    if (lds_strided<W>(Nr + tp[x].S.K) > kMaxLds)
+    const int Nr = (int)((img->shape[x] + kChP - 1) / kChP * kChP);
+    if (lds_strided<W>(Nr + tp[x].S.K + 4) + 2 * (tp[x].S.K + 4) * (int)sizeof(W) > kMaxLds ||
+        (Nr / kChP) * kChC > 256 * kChMaxItems)
       return fail(HC_ERR_RESOURCE, "line too long for the fused Hessian");
   }
   const int64_t N = img->shape[nd - 1];
@@ -878,9 +991,12 @@ using namespace hc;
 extern "C" {
 
 int hc_max_resident_line(int w_f64) {
-  // strided passes bound it: 3 lines x (N + K) x 17 columns in LDS
+  // strided passes bound it: 2 lines x (N + K) x 17 columns in LDS and at
+  // most 256 x kChMaxItems eight-output items per work-group
   const int es = w_f64 ? 8 : 4;
-  return kMaxLds / (3 * kChS * es) - 128;
+  const int lds_bound = kMaxLds / (2 * kChS * es) - 136;
+  const int item_bound = 256 * kChMaxItems / kChC * kChP;
+  return lds_bound < item_bound ? lds_bound : item_bound;
 }
 
 int hc_hessian_gaussian(const hc_array* in, const hc_array* H, int nH, const hc_taps* taps_s,
