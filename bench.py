@@ -39,8 +39,15 @@ def _parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-edge", type=int, default=96,
                    help="edge of the cubic crop timed on the CPU")
-    p.add_argument("--only", choices=["all", "frangi", "correlate"], default="all",
-                   help="restrict the run (profiler passes)")
+    p.add_argument("--only", choices=["all", "frangi", "correlate", "copy"], default="all",
+                   help="restrict the run (profiler passes); 'copy' runs the plain copy "
+                        "kernel that calibrates the HBM byte counters")
+    p.add_argument("--no-check", action="store_true",
+                   help="skip the fused-vs-reference-order cross-check")
+    p.add_argument("--corr-sizes", default="2048,8192",
+                   help="image edges of the correlate1d / Gaussian measurements")
+    p.add_argument("--summarize", metavar="DIR",
+                   help="condense the rocprofv3 CSV output under DIR and exit")
     return p.parse_args()
 
 
@@ -78,7 +85,7 @@ def per_kernel_stats(records):
     return {k: (len(v), sum(v) / len(v), sum(v)) for k, v in agg.items()}
 
 
-def bench_frangi(steps, warmup):
+def bench_frangi(steps, warmup, check=True):
     import numpy as np
     import torch
 
@@ -103,6 +110,8 @@ def bench_frangi(steps, warmup):
             step(i)
         torch.cuda.synchronize()
     stats = per_kernel_stats(kt.records)
+    if not check:
+        return vol, plan, ms, stats, None, None
     # fused schedule vs the reference's pass order, both on the GPU
     ref = torch.empty_like(vol)
     ridges._frangi_sequential(vol, list(SIGMAS), 0.5, 0.5, None, False, "reflect", 0, ref)
@@ -114,56 +123,159 @@ def bench_frangi(steps, warmup):
     return vol, plan, ms, stats, rel, mx
 
 
-def bench_correlate():
-    """correlate1d (axis -1 and 0) and the fused 2-D gaussian on 2048^2
-    float32, K = 17; rotating buffers totalling > 256 MB."""
+def bench_correlate(sizes=(2048, 8192)):
+    """correlate1d along the contiguous (-1) and the strided (0) axis, and the
+    2-D Gaussian of skimage.filters.gaussian (sigma 2, reflect, truncate 4,
+    preserve_range) on float32 images of 2048^2 (the Gaussian workload) and
+    8192^2, K = 17; rotating buffers totalling > 256 MB so the Infinity Cache
+    does not serve the reads.  GB/s count algorithmic bytes (in + out once)."""
     import numpy as np
     import torch
 
     from cucim_amd import _array, _lib
+    from cucim_amd.skimage import filters
     from cucim_amd.skimage._vendored import _ndimage_filters as F
 
-    n = 2048
-    nbuf = 12  # 12 x (16 + 16) MiB = 384 MiB
-    g = torch.Generator(device="cuda")
-    g.manual_seed(0)
-    ins = [torch.rand((n, n), generator=g, device="cuda") for _ in range(nbuf)]
-    outs = [torch.empty((n, n), device="cuda") for _ in range(nbuf)]
+    res = {}
+    lib = _lib.lib()
+    st = _array.stream()
     w = F._gaussian_kernel1d(2.0, 0, 8).astype(np.float32)
     taps = F._Taps(w, 8, np.float32)
-    lib = _lib.lib()
-    hin = [_array.hc_array(t) for t in ins]
-    hout = [_array.hc_array(t) for t in outs]
-    st = _array.stream()
-    res = {}
-    for axis, key in ((1, "correlate1d_axis-1"), (0, "correlate1d_axis0")):
-        def run(i, axis=axis):
+    for n in sizes:
+        nbuf = max(2, (384 << 20) // (n * n * 8))
+        g = torch.Generator(device="cuda")
+        g.manual_seed(0)
+        ins = [torch.rand((n, n), generator=g, device="cuda") for _ in range(nbuf)]
+        outs = [torch.empty((n, n), device="cuda") for _ in range(nbuf)]
+        hin = [_array.hc_array(t) for t in ins]
+        hout = [_array.hc_array(t) for t in outs]
+        algo_bytes = n * n * 8
+        for axis, key in ((1, "correlate1d_axis-1"), (0, "correlate1d_axis0")):
+            def run(i, axis=axis):
+                k = i % nbuf
+                lib.hc_correlate1d(hin[k], hout[k], taps.struct, axis, 0, 0.0, 0, st)
+
+            for i in range(nbuf):
+                run(i)
+            with _lib.KernelTimer() as kt:
+                ms_wall = _events_time(run, 8 * nbuf)
+            st_ = per_kernel_stats(kt.records)
+            kname, (cnt, avg, _) = max(st_.items(), key=lambda kv: kv[1][2])
+            res[f"{key}_{n}"] = {"kernel": kname, "avg_ms": avg,
+                                 "GBps": algo_bytes / (avg * 1e-3) / 1e9,
+                                 "wall_ms_per_call": ms_wall}
+
+        def run_g(i):
             k = i % nbuf
-            lib.hc_correlate1d(hin[k], hout[k], taps.struct, axis, 0, 0.0, 0, st)
+            filters.gaussian(ins[k], sigma=2.0, mode="reflect", truncate=4.0,
+                             preserve_range=True, out=outs[k])
 
         for i in range(nbuf):
-            run(i)
+            run_g(i)
         with _lib.KernelTimer() as kt:
-            ms_wall = _events_time(run, 8 * nbuf)
-        st_ = per_kernel_stats(kt.records)
-        kname, (cnt, avg, _) = max(st_.items(), key=lambda kv: kv[1][2])
-        res[key] = {"kernel": kname, "avg_ms": avg, "GBps": n * n * 8 / (avg * 1e-3) / 1e9,
-                    "wall_ms_per_call": ms_wall}
-
-    def run2(i):
-        k = i % nbuf
-        lib.hc_separable2d(hin[k], hout[k], taps.struct, taps.struct, 0, 0, 0.0, st)
-
-    for i in range(nbuf):
-        run2(i)
-    with _lib.KernelTimer() as kt:
-        ms_wall = _events_time(run2, 8 * nbuf)
-    st_ = per_kernel_stats(kt.records)
-    kname, (cnt, avg, _) = max(st_.items(), key=lambda kv: kv[1][2])
-    res["gaussian2d_fused"] = {"kernel": kname, "avg_ms": avg,
-                               "GBps": n * n * 8 / (avg * 1e-3) / 1e9,
-                               "wall_ms_per_call": ms_wall}
+            ms_wall = _events_time(run_g, 8 * nbuf)
+        kern_ms = sum(m for _, m in kt.records) / (8 * nbuf)
+        res[f"gaussian2d_{n}"] = {
+            "kernels": sorted(set(k for k, _ in kt.records)),
+            "kernel_ms_per_call": kern_ms,
+            "GBps_kernels": algo_bytes / (kern_ms * 1e-3) / 1e9,
+            "wall_ms_per_call": ms_wall,
+            "GBps_wall": algo_bytes / (ms_wall * 1e-3) / 1e9}
+        del ins, outs, hin, hout
     return res
+
+
+COPY_BYTES = 256 << 20  # per direction
+
+
+def bench_copy(reps=4):
+    """The plain copy kernel (``k_copy_vec``, reached through ``hc_copy_cast``) over
+    256 MiB -> 256 MiB: a kernel of known HBM traffic that calibrates the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    from cucim_amd import _array, _lib
+
+    n = COPY_BYTES // 4
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        _array.copy_cast(src, dst)
+    with _lib.KernelTimer() as kt:
+        for _ in range(reps):
+            _array.copy_cast(src, dst)
+        torch.cuda.synchronize()
+    ms = sum(m for _, m in kt.records) / len(kt.records)
+    return {"kernel": "k_copy_vec", "bytes_each_way": COPY_BYTES, "avg_ms": ms,
+            "GBps": 2 * COPY_BYTES / (ms * 1e-3) / 1e9}
+
+
+def summarize(root):
+    """Condense rocprofv3 CSV output (``--output-format csv``) under ``root``:
+    kernel statistics, and per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) scaled
+    by what the same counter reads for the plain copy kernel of known size."""
+    import collections
+    import csv
+    import glob
+
+    lines = []
+    for f in sorted(glob.glob(os.path.join(root, "**", "*kernel_stats.csv"), recursive=True)):
+        lines.append(f"# kernel stats: {os.path.relpath(f, root)}")
+        with open(f) as fh:
+            rows = list(csv.reader(fh))
+        lines.append("Name,Calls,TotalDurationNs,AverageNs,Percentage,MinNs,MaxNs")
+        for row in rows[1:]:
+            if row and (row[0].startswith("void hc::") or row[0].startswith("hc::")):
+                lines.append(",".join(row[:7]))
+    counters = collections.defaultdict(list)
+    for f in sorted(glob.glob(os.path.join(root, "**", "*counter_collection.csv"),
+                              recursive=True)):
+        with open(f) as fh:
+            for row in csv.DictReader(fh):
+                name = row.get("Kernel_Name", "?")
+                short = name.split("(")[0].replace("void ", "").split("<")[0]
+                try:
+                    counters[(short, row.get("Counter_Name", "?"))].append(
+                        float(row.get("Counter_Value", "nan")))
+                except ValueError:
+                    pass
+    if counters:
+        known_kib = COPY_BYTES / 1024
+        factor = {}
+        for c in ("FETCH_SIZE", "WRITE_SIZE"):
+            v = counters.get(("hc::k_copy_vec", c))
+            factor[c] = known_kib / (sum(v) / len(v)) if v else None
+        lines.append("# counters (KiB per dispatch, mean over dispatches); calibration: "
+                     f"k_copy_vec moves {COPY_BYTES} B each way")
+        lines.append("# scale factors (known / counter): " + json.dumps(factor))
+        traffic = {}
+        for (k, c), v in sorted(counters.items()):
+            if not k.startswith("hc::"):
+                continue
+            mean = sum(v) / len(v)
+            lines.append(f"{k}, {c}, dispatches={len(v)}, mean_KiB={mean:.1f}")
+            if factor.get(c):
+                traffic.setdefault(k.replace("hc::", ""), {})[c] = mean * 1024 * factor[c]
+        for k, d in traffic.items():
+            if len(d) == 2:
+                d["bytes"] = d["FETCH_SIZE"] + d["WRITE_SIZE"]
+        lines.append("TRAFFIC " + json.dumps(traffic))
+    return "\n".join(lines)
+
+
+def read_traffic(kernel, profile):
+    """HBM bytes per launch of ``kernel`` from a committed profile summary
+    (the ``TRAFFIC`` line written by ``--summarize``), or None."""
+    path = os.path.join(ROOT, "profiles", profile)
+    try:
+        with open(path) as fh:
+            for ln in fh:
+                if ln.startswith("TRAFFIC "):
+                    d = json.loads(ln[len("TRAFFIC "):])
+                    return d.get(kernel, {}).get("bytes")
+    except OSError:
+        return None
+    return None
 
 
 def cpu_baseline(vol, edge):
@@ -194,19 +306,40 @@ def cpu_baseline(vol, edge):
         {"sample": f"{edge}^3 crop", "rel_l2_vs_fp64": rel, "max_abs_vs_fp64": mx}
 
 
+def _corr_roofline(corr):
+    """HBM roofline of the correlate1d line kernel (contiguous axis, 8192^2
+    float32, K = 17): algorithmic bytes (in + out) per launch over its
+    average launch time."""
+    if not corr or "correlate1d_axis-1_8192" not in corr:
+        return None
+    c = corr["correlate1d_axis-1_8192"]
+    return {"bound": "hbm", "kernel": c["kernel"], "achieved": c["GBps"],
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": c["GBps"] / HBM_PEAK_GBS,
+            "traffic": read_traffic(c["kernel"], "correlate1d_8192sq_counters.txt"),
+            "algorithmic_bytes": 8192 * 8192 * 8, "avg_launch_ms": c["avg_ms"]}
+
+
 def main():
     args = _parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.summarize:
+        print(summarize(args.summarize))
+        return
     import torch
 
     torch.cuda.set_device(0)
-    if args.only == "correlate":
-        print(json.dumps({"correlate1d": bench_correlate()}))
+    if args.only == "copy":
+        print(json.dumps({"copy": bench_copy()}))
         return
-    vol, plan, ms, stats, rel_fused, max_fused = bench_frangi(args.steps, args.warmup)
+    if args.only == "correlate":
+        sizes = tuple(int(v) for v in args.corr_sizes.split(","))
+        print(json.dumps({"copy": bench_copy(), "correlate1d": bench_correlate(sizes)}))
+        return
+    vol, plan, ms, stats, rel_fused, max_fused = bench_frangi(args.steps, args.warmup,
+                                                               not args.no_check)
     nvox = math.prod(SHAPE)
     value = nvox / (ms * 1e-3)
     # dominant kernel of the step by total time
@@ -224,7 +357,9 @@ def main():
     else:
         fl = 0.0
     achieved = fl / (avg_ms * 1e-3) / 1e12 if fl else None
-    corr = bench_correlate() if args.only == "all" else None
+    corr = None
+    if args.only == "all":
+        corr = bench_correlate(tuple(int(v) for v in args.corr_sizes.split(",")))
     cpu, parity = (None, None)
     if not args.no_cpu and args.only == "all":
         cpu, parity = cpu_baseline(vol, args.cpu_edge)
@@ -252,13 +387,14 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": (achieved / FP32_PEAK_TFLOPS) if achieved else None,
-            "traffic": None,
+            "traffic": read_traffic(dom, "frangi_256cubed_counters.txt"),
             "note": "FP32 FMA sweeps of the separable passes; packed FP32 is disabled in "
                     "this build, so the reachable VALU ceiling is "
                     f"{FP32_VALU_NOPACK_TFLOPS} TFLOP/s",
             "avg_launch_ms": avg_ms,
             "launches_per_step": cnt / args.steps,
         },
+        "roofline_correlate1d": _corr_roofline(corr),
         "kernels_ms_per_step": {k: v[2] / args.steps for k, v in stats.items()},
         "frangi_flops_per_voxel": flops_vox,
         "frangi_tflops": flops_vox * nvox / (ms * 1e-3) / 1e12,

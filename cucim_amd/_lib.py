@@ -24,6 +24,9 @@ MODE_REFLECT, MODE_MIRROR, MODE_NEAREST, MODE_WRAP, MODE_CONSTANT = range(5)
 # hc_algorithm
 ALGO_AUTO, ALGO_TILED, ALGO_GATHER = range(3)
 
+# hc_tune_knob
+TUNE_CONTIG_SEG, TUNE_SEP2D_TY = 0, 1
+
 # hc_status
 OK, ERR_VALUE, ERR_RESOURCE, ERR_HIP, ERR_UNSUPPORTED = range(5)
 
@@ -34,6 +37,8 @@ ENTRY_POINTS = (
     "hc_max_resident_line",
     "hc_timing_enable",
     "hc_timing_collect",
+    "hc_tune_set",
+    "hc_tune_get",
     "hc_correlate1d",
     "hc_correlate_nd",
     "hc_separable2d",
@@ -92,6 +97,8 @@ _SIGNATURES = {
     "hc_max_resident_line": ([_i], _i),
     "hc_timing_enable": ([_i], _i),
     "hc_timing_collect": ([ctypes.c_char_p, _i, _P(_d), _i], _i),
+    "hc_tune_set": ([_i, _i], _i),
+    "hc_tune_get": ([_i], _i),
     "hc_correlate1d": ([_A, _A, _T, _i, _i, _d, _i, _v], _i),
     "hc_correlate_nd": ([_A, _A, _v, _P(ctypes.c_int64), _P(ctypes.c_int32),
                          _P(ctypes.c_int32), _i, _d, _v], _i),

@@ -15,6 +15,10 @@ void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int hip_check(hipError_t e, const char* what);
 
+// ------------------------------------------------------------ tuning --
+// Current value of a tile-shape knob (hc_tune_knob); 0 = built-in heuristic.
+int tune_get(int knob);
+
 // ------------------------------------------------------------- tracing --
 // When enabled with hc_timing_enable(1), every kernel launch is bracketed by
 // a pair of hipEvents on its stream and tagged with the kernel's name;
@@ -80,7 +84,7 @@ __device__ __forceinline__ W load_w(const void* __restrict__ base, int64_t i) {
 }
 
 template <typename W>
-__device__ __forceinline__ constexpr int w_dtype() {
+__host__ __device__ __forceinline__ constexpr int w_dtype() {
   return sizeof(W) == 4 ? HC_FLOAT32 : HC_FLOAT64;
 }
 

@@ -1,4 +1,5 @@
 // Host-side helpers of the C ABI: error state, array checks, tap packing.
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -79,6 +80,9 @@ int make_taps(const hc_taps* t, int pad_to, TapPack<W>* out) {
   return HC_OK;
 }
 
+std::atomic<int> g_tune[HC_TUNE_NKNOBS];
+int tune_get(int knob) { return g_tune[knob].load(std::memory_order_relaxed); }
+
 namespace {
 struct TimingRec {
   const char* name;
@@ -127,6 +131,22 @@ extern "C" {
 const char* hc_last_error(void) { return hc::g_last_error.c_str(); }
 
 int hc_version(void) { return 1; }
+
+int hc_tune_set(int knob, int value) {
+  if (knob < 0 || knob >= HC_TUNE_NKNOBS) return hc::fail(HC_ERR_VALUE, "unknown tuning knob");
+  if (knob == HC_TUNE_CONTIG_SEG && value != 0 &&
+      (value < 256 || value > 4096 || value % 256 != 0))
+    return hc::fail(HC_ERR_VALUE, "segment must be a multiple of 256 in [256, 4096]");
+  if (knob == HC_TUNE_SEP2D_TY && value != 0 && value != 16 && value != 32 && value != 64)
+    return hc::fail(HC_ERR_VALUE, "tile rows must be 16, 32 or 64");
+  hc::g_tune[knob].store(value);
+  return HC_OK;
+}
+
+int hc_tune_get(int knob) {
+  if (knob < 0 || knob >= HC_TUNE_NKNOBS) return -1;
+  return hc::g_tune[knob].load();
+}
 
 int hc_timing_enable(int on) {
   std::lock_guard<std::mutex> lk(hc::g_tmu);

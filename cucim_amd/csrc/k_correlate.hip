@@ -105,22 +105,92 @@ struct TiledArgs {
   int64_t N;       // line length
   int64_t inner;   // element stride along a line (1 = contiguous)
   int64_t nseg;    // segments per line
+  int seg;         // contiguous: outputs per wave segment (multiple of 256)
   int mode;
   W cval;
   TapPack<W> tp;   // K padded to a multiple of 4
 };
 
-constexpr int kSeg = 1024;       // contiguous: outputs per wave segment
 constexpr int kStrCols = 64;     // strided: lines per work-group
 constexpr int kStrStride = 65;   // strided: LDS row stride (elements)
 constexpr int kStrTA = 64;       // strided: outputs per line per work-group
 
+// 16-byte loads each lane keeps in flight while staging.
+constexpr int kVecBatch = 8;
+
+// 16-byte vectors of the compute type.
+template <typename W> struct Vec16;
+template <> struct Vec16<float> {
+  typedef float T __attribute__((ext_vector_type(4)));
+  static constexpr int n = 4;
+};
+template <> struct Vec16<double> {
+  typedef double T __attribute__((ext_vector_type(2)));
+  static constexpr int n = 2;
+};
+
+// ext[s] = row[p0 + s] (boundary-extended) for s in [0, L), one wave.  Input
+// chunks of Vec16 elements aligned to the row start are read with one
+// 16-byte load when they lie inside the row, element-wise through the
+// boundary remap otherwise.  Requires row 16-byte aligned.
 template <typename W>
+__device__ __forceinline__ bool stage_row_vec(W* __restrict__ ext, int L, int lane,
+                                              const W* __restrict__ row, int64_t N, int64_t p0,
+                                              int mode, W cval) {
+  constexpr int VN = Vec16<W>::n;
+  typedef typename Vec16<W>::T V;
+  const int64_t a0 = p0 >= 0 ? (p0 / VN) * VN : -((-p0 + VN - 1) / VN) * VN;
+  const int shift = (int)(p0 - a0);
+  const int nc = (shift + L + VN - 1) / VN;
+  bool bad = false;
+  for (int c0 = lane; c0 < nc; c0 += 64 * kVecBatch) {
+    W v[kVecBatch][VN];
+#pragma unroll
+    for (int u = 0; u < kVecBatch; ++u) {
+      const int c = c0 + 64 * u;
+      const int64_t g = a0 + (int64_t)c * VN;
+#pragma unroll
+      for (int j = 0; j < VN; ++j) v[u][j] = cval;
+      if (c < nc) {
+        if (g >= 0 && g + VN <= N) {
+          const V t = *reinterpret_cast<const V*>(row + g);
+#pragma unroll
+          for (int j = 0; j < VN; ++j) v[u][j] = t[j];
+        } else {
+#pragma unroll
+          for (int j = 0; j < VN; ++j) {
+            const int64_t r = remap(g + j, N, mode);
+            if (r >= 0) v[u][j] = row[r];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kVecBatch; ++u) {
+      const int c = c0 + 64 * u;
+      if (c < nc) {
+#pragma unroll
+        for (int j = 0; j < VN; ++j) {
+          const int sl = c * VN + j - shift;
+          if (sl >= 0 && sl < L) {
+            ext[sl] = v[u][j];
+            bad |= !finite(v[u][j]);
+          }
+        }
+      }
+    }
+  }
+  return bad;
+}
+
+// VEC: input and output have the compute dtype, 16-byte aligned rows
+// (line length a multiple of 16 bytes): vector loads and stores.
+template <typename W, bool VEC>
 __global__ __launch_bounds__(256) void k_tiled_contig(const TiledArgs<W> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   W* smem = reinterpret_cast<W*>(smem_raw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int L = kSeg + a.tp.K + 4;  // conv_row4 reads 8 past the last group
+  const int L = a.seg + a.tp.K + 4;  // conv_row4 reads 8 past the last group
   W* ext = smem + wv * L;
   W* wl = smem + 4 * L;
   stage_taps<W>(wl, a.tp, threadIdx.x, 256);
@@ -131,37 +201,57 @@ __global__ __launch_bounds__(256) void k_tiled_contig(const TiledArgs<W> a) {
   bool bad = false;
   if (active) {
     const int64_t line = seg / a.nseg;
-    s0 = (seg - line * a.nseg) * kSeg;
+    s0 = (seg - line * a.nseg) * a.seg;
     lbase = line * a.N;
     const int64_t p0 = s0 - a.tp.anchor;
     const int64_t N = a.N, lb = lbase;
     const int mode = a.mode;
-    bad = stage_any<W>(ext, L, lane, 64, a.in, a.in_dt, a.cval, false,
-                       [=](int e, int& slot) -> int64_t {
-                         slot = e;
-                         const int64_t r = remap(p0 + e, N, mode);
-                         return r < 0 ? -1 : lb + r;
-                       });
+    if (VEC)
+      bad = stage_row_vec<W>(ext, L, lane, reinterpret_cast<const W*>(a.in) + lb, N, p0, mode,
+                             a.cval);
+    else
+      bad = stage_any<W>(ext, L, lane, 64, a.in, a.in_dt, a.cval, false,
+                         [=](int e, int& slot) -> int64_t {
+                           slot = e;
+                           const int64_t r = remap(p0 + e, N, mode);
+                           return r < 0 ? -1 : lb + r;
+                         });
   }
   const bool skip = __syncthreads_or(bad);
   if (!active) return;
   with_skip(skip, [&](auto sk) {
     constexpr bool SK = decltype(sk)::value;
 #pragma unroll 1
-    for (int g = 0; g < kSeg / 256; ++g) {
+    for (int g = 0; g < a.seg / 256; ++g) {
       const int o0 = 256 * g + 4 * lane;
       const int64_t o = s0 + o0;
       if (o >= a.N) break;
       W acc[4] = {(W)0, (W)0, (W)0, (W)0};
       conv_row4<SK>(acc, ext + o0, wl, a.tp);
+      if (VEC && o + 4 <= a.N) {
+        W* op = reinterpret_cast<W*>(a.out) + lbase + o;
+        if (sizeof(W) == 4) {
+          typename Vec16<float>::T t = {(float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]};
+          *reinterpret_cast<typename Vec16<float>::T*>(op) = t;
+        } else {
+          typename Vec16<double>::T t0 = {(double)acc[0], (double)acc[1]};
+          typename Vec16<double>::T t1 = {(double)acc[2], (double)acc[3]};
+          reinterpret_cast<typename Vec16<double>::T*>(op)[0] = t0;
+          reinterpret_cast<typename Vec16<double>::T*>(op)[1] = t1;
+        }
+      } else {
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (o + j < a.N) store_cast<W>(a.out, lbase + o + j, a.out_dt, acc[j]);
+        for (int j = 0; j < 4; ++j)
+          if (o + j < a.N) store_cast<W>(a.out, lbase + o + j, a.out_dt, acc[j]);
+      }
     }
   });
 }
 
-template <typename W>
+// VEC: input and output have the compute dtype, the element stride of the
+// line (inner) is a multiple of 16 bytes and the data 16-byte aligned: each
+// 16-byte load brings one position of Vec16 neighbouring lines.
+template <typename W, bool VEC>
 __global__ __launch_bounds__(256) void k_tiled_strided(const TiledArgs<W> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   W* ext = reinterpret_cast<W*>(smem_raw);
@@ -182,14 +272,55 @@ __global__ __launch_bounds__(256) void k_tiled_strided(const TiledArgs<W> a) {
   {
     const int64_t N = a.N, inner = a.inner;
     const int mode = a.mode;
-    bad = stage_any<W>(ext, R * 64, threadIdx.x, 256, a.in, a.in_dt, a.cval, false,
-                       [=](int e, int& slot) -> int64_t {
-                         const int q = e >> 6, c = e & 63;  // c == lane
-                         slot = q * kStrStride + c;
-                         if (!lok) return -1;
-                         const int64_t r = remap(p0 + q, N, mode);
-                         return r < 0 ? -1 : lbase + r * inner;
-                       });
+    if (VEC) {
+      constexpr int VN = Vec16<W>::n, CPR = 64 / VN;  // chunks per LDS row
+      typedef typename Vec16<W>::T V;
+      const W* in = reinterpret_cast<const W*>(a.in);
+      const int64_t Lt = lb * kStrCols;
+      const int total = R * CPR;
+      bad = false;
+      for (int e0 = threadIdx.x; e0 < total; e0 += 256 * kVecBatch) {
+        W v[kVecBatch][VN];
+#pragma unroll
+        for (int u = 0; u < kVecBatch; ++u) {
+          const int e = e0 + 256 * u;
+          const int q = e / CPR, c = (e - q * CPR) * VN;
+#pragma unroll
+          for (int j = 0; j < VN; ++j) v[u][j] = a.cval;
+          const int64_t Lc = Lt + c;
+          if (e < total && Lc < a.nlines) {
+            const int64_t r = remap(p0 + q, N, mode);
+            if (r >= 0) {
+              const int64_t lo2 = Lc / inner, li2 = Lc - lo2 * inner;
+              const V t = *reinterpret_cast<const V*>(in + lo2 * N * inner + li2 + r * inner);
+#pragma unroll
+              for (int j = 0; j < VN; ++j) v[u][j] = t[j];
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kVecBatch; ++u) {
+          const int e = e0 + 256 * u;
+          if (e < total) {
+            const int q = e / CPR, c = (e - q * CPR) * VN;
+#pragma unroll
+            for (int j = 0; j < VN; ++j) {
+              ext[q * kStrStride + c + j] = v[u][j];
+              bad |= !finite(v[u][j]);
+            }
+          }
+        }
+      }
+    } else {
+      bad = stage_any<W>(ext, R * 64, threadIdx.x, 256, a.in, a.in_dt, a.cval, false,
+                         [=](int e, int& slot) -> int64_t {
+                           const int q = e >> 6, c = e & 63;  // c == lane
+                           slot = q * kStrStride + c;
+                           if (!lok) return -1;
+                           const int64_t r = remap(p0 + q, N, mode);
+                           return r < 0 ? -1 : lbase + r * inner;
+                         });
+    }
   }
   const bool skip = __syncthreads_or(bad);
   if (!lok) return;
@@ -201,9 +332,16 @@ __global__ __launch_bounds__(256) void k_tiled_strided(const TiledArgs<W> a) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) acc[j] = (W)0;
     conv_col16<SK, W, kStrStride>(acc, ext + (16 * wv) * kStrStride + lane, wl, a.tp);
+    if (VEC) {
+      W* op = reinterpret_cast<W*>(a.out) + lbase;
 #pragma unroll
-    for (int j = 0; j < 16; ++j)
-      if (n0 + j < a.N) store_cast<W>(a.out, lbase + (n0 + j) * a.inner, a.out_dt, acc[j]);
+      for (int j = 0; j < 16; ++j)
+        if (n0 + j < a.N) op[(n0 + j) * a.inner] = acc[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (n0 + j < a.N) store_cast<W>(a.out, lbase + (n0 + j) * a.inner, a.out_dt, acc[j]);
+    }
   });
 }
 
@@ -223,7 +361,9 @@ struct Sep2dArgs {
   TapPack<W> t0, t1;
 };
 
-template <typename W, int TY>
+// VEC: typed input/output, rows 16-byte aligned: the tile is staged with
+// 16-byte loads (aligned to the row start) and written with 16-byte stores.
+template <typename W, int TY, bool VEC>
 __global__ __launch_bounds__(256) void k_sep2d(const Sep2dArgs<W> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   W* A = reinterpret_cast<W*>(smem_raw);
@@ -243,13 +383,67 @@ __global__ __launch_bounds__(256) void k_sep2d(const Sep2dArgs<W> a) {
     {
       const int64_t H = a.H, Wd = a.Wd, ya = y0 - a.t0.anchor, xa = x0 - a.t1.anchor;
       const int m0 = a.mode0, m1 = a.mode1;
-      bad = stage_any<W>(A, RY * kSepCols, threadIdx.x, 256, a.in, a.in_dt, a.cval, false,
-                         [=](int e, int& slot) -> int64_t {
-                           const int q = e >> 7, c = e & (kSepCols - 1);
-                           slot = c < CX ? e : -1;
-                           const int64_t y = remap(ya + q, H, m0), x = remap(xa + c, Wd, m1);
-                           return (y < 0 || x < 0) ? -1 : y * Wd + x;
-                         });
+      if (VEC) {
+        constexpr int VN = Vec16<W>::n;
+        typedef typename Vec16<W>::T V;
+        const W* in = reinterpret_cast<const W*>(a.in);
+        const int64_t ax0 = xa >= 0 ? (xa / VN) * VN : -((-xa + VN - 1) / VN) * VN;
+        const int shift = (int)(xa - ax0);
+        const int cpr = (shift + CX + VN - 1) / VN;  // chunks per tile row
+        const int total = RY * cpr;
+        bad = false;
+        for (int e0 = threadIdx.x; e0 < total; e0 += 256 * kVecBatch) {
+          W v[kVecBatch][VN];
+#pragma unroll
+          for (int u = 0; u < kVecBatch; ++u) {
+            const int e = e0 + 256 * u;
+#pragma unroll
+            for (int j = 0; j < VN; ++j) v[u][j] = a.cval;
+            if (e < total) {
+              const int q = e / cpr, cc = e - q * cpr;
+              const int64_t y = remap(ya + q, H, m0);
+              const int64_t gx = ax0 + (int64_t)cc * VN;
+              if (y >= 0) {
+                const W* row = in + y * Wd;
+                if (gx >= 0 && gx + VN <= Wd) {
+                  const V t = *reinterpret_cast<const V*>(row + gx);
+#pragma unroll
+                  for (int j = 0; j < VN; ++j) v[u][j] = t[j];
+                } else {
+#pragma unroll
+                  for (int j = 0; j < VN; ++j) {
+                    const int64_t x = remap(gx + j, Wd, m1);
+                    if (x >= 0) v[u][j] = row[x];
+                  }
+                }
+              }
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < kVecBatch; ++u) {
+            const int e = e0 + 256 * u;
+            if (e < total) {
+              const int q = e / cpr, cc = e - q * cpr;
+#pragma unroll
+              for (int j = 0; j < VN; ++j) {
+                const int c = cc * VN + j - shift;
+                if (c >= 0 && c < CX) {
+                  A[q * kSepCols + c] = v[u][j];
+                  bad |= !finite(v[u][j]);
+                }
+              }
+            }
+          }
+        }
+      } else {
+        bad = stage_any<W>(A, RY * kSepCols, threadIdx.x, 256, a.in, a.in_dt, a.cval, false,
+                           [=](int e, int& slot) -> int64_t {
+                             const int q = e >> 7, c = e & (kSepCols - 1);
+                             slot = c < CX ? e : -1;
+                             const int64_t y = remap(ya + q, H, m0), x = remap(xa + c, Wd, m1);
+                             return (y < 0 || x < 0) ? -1 : y * Wd + x;
+                           });
+      }
     }
     const bool skip0 = __syncthreads_or(bad);
     // pass 1 (axis 0) over every staged column, rounded to the output dtype
@@ -283,9 +477,23 @@ __global__ __launch_bounds__(256) void k_sep2d(const Sep2dArgs<W> a) {
         if (y >= a.H || x >= a.Wd) continue;
         W acc[4] = {(W)0, (W)0, (W)0, (W)0};
         conv_row4<SK>(acc, U + i * kSepCols + o, w1l, a.t1);
+        if (VEC && x + 4 <= a.Wd) {
+          W* op = reinterpret_cast<W*>(a.out) + y * a.Wd + x;
+          if (sizeof(W) == 4) {
+            typename Vec16<float>::T t = {(float)acc[0], (float)acc[1], (float)acc[2],
+                                          (float)acc[3]};
+            *reinterpret_cast<typename Vec16<float>::T*>(op) = t;
+          } else {
+            typename Vec16<double>::T t0 = {(double)acc[0], (double)acc[1]};
+            typename Vec16<double>::T t1 = {(double)acc[2], (double)acc[3]};
+            reinterpret_cast<typename Vec16<double>::T*>(op)[0] = t0;
+            reinterpret_cast<typename Vec16<double>::T*>(op)[1] = t1;
+          }
+        } else {
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (x + j < a.Wd) store_cast<W>(a.out, y * a.Wd + x + j, a.out_dt, acc[j]);
+          for (int j = 0; j < 4; ++j)
+            if (x + j < a.Wd) store_cast<W>(a.out, y * a.Wd + x + j, a.out_dt, acc[j]);
+        }
       }
     });
     __syncthreads();
@@ -345,6 +553,17 @@ int launch_gather_1d(const hc_array* in, const hc_array* out, const hc_taps* tap
   return HC_OK;
 }
 
+// Outputs per wave segment of the contiguous-axis kernel: the tuned value
+// (hc_tune_set(HC_TUNE_CONTIG_SEG, ...)) or, by default, the longest of
+// 1024 / 512 / 256 that still gives 16 waves per CU.
+int contig_segment(int64_t N, int64_t nlines) {
+  const int t = tune_get(HC_TUNE_CONTIG_SEG);
+  if (t > 0) return t;
+  int seg = 1024;
+  while (seg > 256 && nlines * ((N + seg - 1) / seg) < 16 * 256) seg /= 2;
+  return seg;
+}
+
 template <typename W>
 int launch_tiled_1d(const hc_array* in, const hc_array* out, const hc_taps* taps, int axis,
                     int mode, double cval, hipStream_t st) {
@@ -363,13 +582,30 @@ int launch_tiled_1d(const hc_array* in, const hc_array* out, const hc_taps* taps
   a.cval = (W)cval;
   if (numel(in) == 0) return HC_OK;
   if (ln.inner == 1) {
+    const bool vec_contig = in->dtype == w_dtype<W>() && out->dtype == w_dtype<W>() &&
+                            (ln.N * (int64_t)sizeof(W)) % 16 == 0 &&
+                            (((uintptr_t)in->data | (uintptr_t)out->data) % 16) == 0 &&
+                            is_c_contiguous(in) && is_c_contiguous(out);
     a.nlines = ln.outer;
-    a.nseg = (ln.N + kSeg - 1) / kSeg;
+    a.seg = contig_segment(ln.N, a.nlines);
+    a.nseg = (ln.N + a.seg - 1) / a.seg;
     const int64_t nsegs = a.nlines * a.nseg;
     const int64_t blocks = (nsegs + 3) / 4;
     if (blocks > 0x7fffffff) return fail(HC_ERR_VALUE, "array too large");
-    const size_t smem = ((size_t)4 * (kSeg + a.tp.K + 4) + a.tp.K + 4) * sizeof(W);
-    HC_LAUNCH("k_tiled_contig", st, k_tiled_contig<W>, dim3((unsigned)blocks), dim3(256), smem, a);
+    const size_t smem = ((size_t)4 * (a.seg + a.tp.K + 4) + a.tp.K + 4) * sizeof(W);
+    if (smem > 160 * 1024) return fail(HC_ERR_RESOURCE, "segment does not fit in LDS");
+    if (smem > 64 * 1024) {
+      const hipError_t e = hipFuncSetAttribute(
+          vec_contig ? (const void*)k_tiled_contig<W, true> : (const void*)k_tiled_contig<W, false>,
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+      if (e != hipSuccess) return hip_check(e, "hipFuncSetAttribute(k_tiled_contig)");
+    }
+    if (vec_contig)
+      HC_LAUNCH("k_tiled_contig", st, (k_tiled_contig<W, true>), dim3((unsigned)blocks),
+                dim3(256), smem, a);
+    else
+      HC_LAUNCH("k_tiled_contig", st, (k_tiled_contig<W, false>), dim3((unsigned)blocks),
+                dim3(256), smem, a);
   } else {
     a.nlines = ln.outer * ln.inner;
     a.nseg = (ln.N + kStrTA - 1) / kStrTA;
@@ -378,7 +614,16 @@ int launch_tiled_1d(const hc_array* in, const hc_array* out, const hc_taps* taps
     if (blocks > 0x7fffffff) return fail(HC_ERR_VALUE, "array too large");
     const size_t smem =
         ((size_t)(kStrTA + a.tp.K + 4) * kStrStride + a.tp.K + 4) * sizeof(W);
-    HC_LAUNCH("k_tiled_strided", st, k_tiled_strided<W>, dim3((unsigned)blocks), dim3(256), smem, a);
+    const bool vec = in->dtype == w_dtype<W>() && out->dtype == w_dtype<W>() &&
+                     (ln.inner * (int64_t)sizeof(W)) % 16 == 0 &&
+                     (((uintptr_t)in->data | (uintptr_t)out->data) % 16) == 0 &&
+                     is_c_contiguous(in) && is_c_contiguous(out);
+    if (vec)
+      HC_LAUNCH("k_tiled_strided", st, (k_tiled_strided<W, true>), dim3((unsigned)blocks),
+                dim3(256), smem, a);
+    else
+      HC_LAUNCH("k_tiled_strided", st, (k_tiled_strided<W, false>), dim3((unsigned)blocks),
+                dim3(256), smem, a);
   }
   HC_HIP(hipGetLastError());
   return HC_OK;
@@ -419,10 +664,10 @@ int launch_gather_nd(const hc_array* in, const hc_array* out, const void* w_dev,
   return HC_OK;
 }
 
-template <typename W, int TY>
+template <typename W, int TY, bool VEC>
 int launch_sep2d_ty(const Sep2dArgs<W>& a, hipStream_t st, size_t smem) {
   if (smem > 64 * 1024) {
-    const hipError_t e = hipFuncSetAttribute((const void*)k_sep2d<W, TY>,
+    const hipError_t e = hipFuncSetAttribute((const void*)k_sep2d<W, TY, VEC>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
                                              (int)smem);
     if (e != hipSuccess) {
@@ -431,7 +676,7 @@ int launch_sep2d_ty(const Sep2dArgs<W>& a, hipStream_t st, size_t smem) {
     }
   }
   const int64_t grid = std::min<int64_t>(a.ntiles, 256 * 8);
-  HC_LAUNCH("k_sep2d", st, (k_sep2d<W, TY>), dim3((unsigned)grid), dim3(256), smem, a);
+  HC_LAUNCH("k_sep2d", st, (k_sep2d<W, TY, VEC>), dim3((unsigned)grid), dim3(256), smem, a);
   HC_HIP(hipGetLastError());
   return HC_OK;
 }
@@ -458,14 +703,25 @@ int launch_sep2d(const hc_array* in, const hc_array* out, const hc_taps* t0, con
   a.mode1 = mode1;
   a.cval = (W)cval;
   if (a.H == 0 || a.Wd == 0) return HC_OK;
-  constexpr int TY = sizeof(W) == 4 ? 32 : 16;
+  int TY = tune_get(HC_TUNE_SEP2D_TY);
+  if (TY != 16 && TY != 32 && TY != 64) TY = sizeof(W) == 4 ? 32 : 16;
   a.ntx = (a.Wd + a.TX - 1) / a.TX;
   const int64_t nty = (a.H + TY - 1) / TY;
   a.ntiles = a.ntx * nty;
   const size_t smem =
       ((size_t)(TY + a.t0.K + 4 + TY) * kSepCols + a.t0.K + a.t1.K + 8) * sizeof(W);
   if (smem > 160 * 1024) return fail(HC_ERR_RESOURCE, "kernels too long for the fused 2-D tile");
-  return launch_sep2d_ty<W, TY>(a, st, smem);
+  const bool vec = in->dtype == w_dtype<W>() && out->dtype == w_dtype<W>() &&
+                   (a.Wd * (int64_t)sizeof(W)) % 16 == 0 &&
+                   (((uintptr_t)in->data | (uintptr_t)out->data) % 16) == 0;
+  if (vec) {
+    if (TY == 16) return launch_sep2d_ty<W, 16, true>(a, st, smem);
+    if (TY == 32) return launch_sep2d_ty<W, 32, true>(a, st, smem);
+    return launch_sep2d_ty<W, 64, true>(a, st, smem);
+  }
+  if (TY == 16) return launch_sep2d_ty<W, 16, false>(a, st, smem);
+  if (TY == 32) return launch_sep2d_ty<W, 32, false>(a, st, smem);
+  return launch_sep2d_ty<W, 64, false>(a, st, smem);
 }
 
 }  // namespace

@@ -97,6 +97,20 @@ int hc_max_resident_line(int w_f64);
 int hc_timing_enable(int on);
 int hc_timing_collect(char* names, int names_len, double* ms, int max_records);
 
+/* Tile-shape knobs (no counterpart in the reference, whose shared-memory
+ * kernel fixes its tiles in filters/_separable_filtering.py:17-91).  A value
+ * of 0 restores the built-in heuristic.  Set by the autotuner in
+ * cucim_amd/autotune.py; process-wide, read at each launch. */
+typedef enum hc_tune_knob {
+  HC_TUNE_CONTIG_SEG = 0,   /* contiguous-axis correlate: outputs per wave
+                               segment, 256..4096, multiple of 256 */
+  HC_TUNE_SEP2D_TY = 1,     /* fused 2-D separable: output rows per tile,
+                               16, 32 or 64 */
+  HC_TUNE_NKNOBS = 8
+} hc_tune_knob;
+int hc_tune_set(int knob, int value);
+int hc_tune_get(int knob);
+
 /* correlate1d / convolve1d core: _vendored/_ndimage_filters.py:271-344
  * (_correlate_or_convolve1d), the shared-memory kernel of
  * filters/_separable_filtering.py:977-1070 and the elementwise kernel of

@@ -73,3 +73,20 @@ def test_argument_errors_return_before_launch():
         _lib.check(_lib.ERR_VALUE)
     with pytest.raises(_lib.ResourceLimitError):
         _lib.check(_lib.ERR_RESOURCE)
+
+
+def test_no_unresolved_library_symbols():
+    """Every symbol of the library's own namespace is defined inside it (an
+    unresolved one only fails when the GPU process loads it)."""
+    import shutil
+    import subprocess
+
+    from cucim_amd import _lib
+
+    nm = shutil.which("nm")
+    if nm is None:
+        pytest.skip("nm not available")
+    out = subprocess.run([nm, "-D", "--undefined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    bad = [ln for ln in out.splitlines() if "_ZN2hc" in ln or " hc_" in ln]
+    assert not bad, bad

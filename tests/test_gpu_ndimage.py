@@ -187,7 +187,11 @@ def test_noncontiguous_input_and_output(gpu):
 ])
 @pytest.mark.parametrize("mode", ["nearest", "reflect", "constant", "wrap", "mirror"])
 @pytest.mark.parametrize("shape", [(64, 57), (300, 200), (7, 9, 11)])
-def test_separable_filters(gpu, fn, kw, mode, shape):
+@pytest.mark.parametrize("fused", [False, True])
+def test_separable_filters(gpu, fn, kw, mode, shape, fused, monkeypatch):
+    from cucim_amd.skimage._vendored import _ndimage_filters as F
+
+    monkeypatch.setattr(F, "FUSED_2D", fused)
     ndi = _ndi()
     x = _img(shape, np.float32)
     if len(shape) == 3 and "sigma" in kw and isinstance(kw["sigma"], tuple):
@@ -240,3 +244,86 @@ def test_errors(gpu):
     with pytest.raises(NotImplementedError):
         ndi.correlate1d(torch.zeros((4, 5), dtype=torch.int32, device="cuda"), [1.0, 2.0],
                         mode="constant", cval=float("nan"))
+
+
+def _same_with_nan(got, exp):
+    """Bit-exact on finite values; NaN where the oracle has NaN (NaN
+    payloads differ between the host and the GPU)."""
+    assert got.dtype == exp.dtype
+    gn, en = np.isnan(got), np.isnan(exp)
+    np.testing.assert_array_equal(gn, en)
+    u = f"u{got.itemsize}"
+    np.testing.assert_array_equal(got[~gn].view(u), exp[~en].view(u))
+
+
+@pytest.mark.parametrize("shape", [(40, 64), (33, 57)])
+@pytest.mark.parametrize("fn,kw", [
+    ("sobel", dict(axis=0)),
+    ("sobel", dict(axis=1)),
+    ("correlate1d", dict(weights=[0.5, 0.0, -0.25, 0.0, 1.0], axis=1)),
+    ("correlate1d", dict(weights=[0.5, 0.0, -0.25, 0.0, 1.0], axis=0)),
+    ("gaussian_filter", dict(sigma=1.0)),
+])
+def test_non_finite_inputs(gpu, shape, fn, kw):
+    """Zero taps are skipped as in the reference, so an inf/NaN input only
+    reaches the outputs whose non-zero taps cover it."""
+    ndi = _ndi()
+    x = _img(shape, np.float32)
+    x[5, 7] = np.inf
+    x[20, 30] = -np.inf
+    x[11, 3] = np.nan
+    exp = getattr(on, fn)(x, mode="reflect", acc="emulate", **kw)
+    from cucim_amd.skimage._vendored import _ndimage_filters as F
+
+    for fused in (False, True):
+        F.FUSED_2D = fused
+        try:
+            _same_with_nan(_run(getattr(ndi, fn), x, mode="reflect", **kw), exp)
+        finally:
+            F.FUSED_2D = False
+
+
+@pytest.mark.parametrize("seg", [256, 512, 2048, 4096])
+def test_tuned_contig_segment(gpu, seg):
+    """Every segment length the tuning knob accepts gives the same bits."""
+    from cucim_amd import _lib
+
+    ndi = _ndi()
+    lib = _lib.lib()
+    x = _img((5, 9000), np.float32)
+    w = _img((17,), np.float32, seed=9)
+    exp = on.correlate1d(x, w, 1, mode="mirror", acc="emulate")
+    _lib.check(lib.hc_tune_set(_lib.TUNE_CONTIG_SEG, seg))
+    try:
+        _exact(_run(ndi.correlate1d, x, w, 1, mode="mirror"), exp)
+    finally:
+        lib.hc_tune_set(_lib.TUNE_CONTIG_SEG, 0)
+
+
+@pytest.mark.parametrize("ty", [16, 32, 64])
+@pytest.mark.parametrize("shape", [(300, 260), (257, 131)])
+def test_tuned_sep2d_rows(gpu, ty, shape, monkeypatch):
+    from cucim_amd import _lib
+    from cucim_amd.skimage._vendored import _ndimage_filters as F
+
+    monkeypatch.setattr(F, "FUSED_2D", True)
+
+    ndi = _ndi()
+    lib = _lib.lib()
+    x = _img(shape, np.float32)
+    exp = on.gaussian_filter(x, 2.0, mode="constant", cval=0.5, acc="emulate")
+    _lib.check(lib.hc_tune_set(_lib.TUNE_SEP2D_TY, ty))
+    try:
+        _exact(_run(ndi.gaussian_filter, x, 2.0, mode="constant", cval=0.5), exp)
+    finally:
+        lib.hc_tune_set(_lib.TUNE_SEP2D_TY, 0)
+
+
+def test_tuning_knob_errors(gpu):
+    from cucim_amd import _lib
+
+    lib = _lib.lib()
+    assert lib.hc_tune_set(_lib.TUNE_CONTIG_SEG, 300) == _lib.ERR_VALUE
+    assert lib.hc_tune_set(_lib.TUNE_SEP2D_TY, 24) == _lib.ERR_VALUE
+    assert lib.hc_tune_set(99, 1) == _lib.ERR_VALUE
+    assert lib.hc_tune_get(_lib.TUNE_CONTIG_SEG) == 0
