@@ -19,7 +19,8 @@ LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libcucim_amd.so")
 OBJ = os.path.join(ROOT, "_build")
 
-SOURCES = ["hc_host.hip", "k_correlate.hip", "k_hessian.hip", "k_pointwise.hip"]
+SOURCES = ["hc_host.hip", "k_correlate.hip", "k_hessian.hip", "k_pointwise.hip",
+           "k_stencil2d.hip", "k_ridges.hip"]
 
 ARCH = "gfx950"
 # Packed FP32 instructions are disabled for every HIP translation unit.

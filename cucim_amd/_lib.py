@@ -55,6 +55,11 @@ ENTRY_POINTS = (
     "hc_square_acc",
     "hc_sqrt_inplace",
     "hc_copy_cast",
+    "hc_batch2d_plan",
+    "hc_batch2d_gray_gauss_sobel",
+    "hc_sato_fold",
+    "hc_meijering_fold",
+    "hc_fill_le",
 )
 
 
@@ -116,6 +121,12 @@ _SIGNATURES = {
     "hc_square_acc": ([_A, _A, _i, _v], _i),
     "hc_sqrt_inplace": ([_A, _v], _i),
     "hc_copy_cast": ([_A, _A, _v], _i),
+    "hc_batch2d_plan": ([_v, _i, _i, _i, _v, ctypes.c_int64], ctypes.c_int64),
+    "hc_batch2d_gray_gauss_sobel": ([_v, _i, _v, ctypes.c_int64, _T, _i, _d, _P(_d), _P(_d),
+                                     _i, _i, _v], _i),
+    "hc_sato_fold": ([_A, _i, _A, _d, _i, _v], _i),
+    "hc_meijering_fold": ([_A, _i, _A, _A, _d, _i, _v, _v], _i),
+    "hc_fill_le": ([_A, _d, _d, _v], _i),
 }
 
 _lock = threading.Lock()

@@ -1,0 +1,177 @@
+"""Batched pipelines over many images of uneven sizes in one launch.
+
+The reference filters one image per call (every skimage function takes one
+array); a caller with many tiles loops in Python, one set of kernel launches
+per tile.  Here a plan binds a list of input and output images, builds a
+work-item table once (``hc_batch2d_plan``), and runs the whole batch as one
+persistent launch that can be captured into a HIP graph and replayed.  Tile
+shapes (output rows per work item) are chosen by measuring the candidates
+(``autotune``); choices are remembered per workload signature.
+
+GrayGaussSobelBatch: per image, ``filters.sobel(filters.gaussian(
+color.rgb2gray(rgb), sigma, mode=gauss_mode, cval=gauss_cval,
+truncate=truncate), mode=edge_mode)`` (reference:
+color/colorconv.py:1037-1095, _shared/filters.py:21-166,
+filters/edges.py:148-241), each output equal to the three separate calls.
+"""
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _array, _lib
+from .skimage._vendored import _ndimage_filters as _F
+from .skimage._vendored._ndimage_util import _mode_code
+
+
+class HCTile2d(ctypes.Structure):
+    _fields_ = [
+        ("in_", ctypes.c_void_p),
+        ("out", ctypes.c_void_p),
+        ("H", ctypes.c_int64),
+        ("W", ctypes.c_int64),
+        ("item0", ctypes.c_int64),
+    ]
+
+
+# workload signature -> chosen item rows
+_TUNED = {}
+
+
+def _time_ms(fn, reps):
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fn()
+    torch.cuda.synchronize()
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+class GrayGaussSobelBatch:
+    """Plan of rgb2gray -> gaussian -> sobel over bound images.
+
+    inputs: list of (H, W, 3) uint8 CUDA tensors (C-contiguous);
+    outputs: list of (H, W) float32 CUDA tensors, or None to allocate.
+    """
+
+    ROWS = (16, 32)
+
+    def __init__(self, inputs, outputs=None, *, sigma=1.5, truncate=4.0, gauss_mode="nearest",
+                 gauss_cval=0.0, edge_mode="reflect", rows=None):
+        if len(inputs) == 0:
+            raise ValueError("no images")
+        self.inputs = []
+        for x in inputs:
+            x = _array.as_tensor(x)
+            if x.dtype != torch.uint8 or x.dim() != 3 or x.shape[2] != 3:
+                raise ValueError("inputs must be (H, W, 3) uint8 images")
+            if not x.is_contiguous():
+                raise ValueError("inputs must be C-contiguous")
+            self.inputs.append(x)
+        if outputs is None:
+            outputs = [_array.empty(tuple(x.shape[:2]), torch.float32) for x in self.inputs]
+        self.outputs = []
+        for x, o in zip(self.inputs, outputs):
+            o = _array.as_tensor(o)
+            if o.dtype != torch.float32 or tuple(o.shape) != tuple(x.shape[:2]) \
+                    or not o.is_contiguous():
+                raise ValueError("outputs must be C-contiguous float32 of the inputs' (H, W)")
+            self.outputs.append(o)
+        if len(self.outputs) != len(self.inputs):
+            raise ValueError("inputs and outputs differ in length")
+        if sigma < 0:
+            raise ValueError("Sigma values less than zero are not valid")
+        radius = int(float(truncate) * float(sigma) + 0.5)
+        w = _F._gaussian_kernel1d(float(sigma), 0, radius).astype(np.float32)
+        self.taps = _F._Taps(w, radius, np.float32)
+        if self.taps.w.size > 33:
+            raise _lib.ResourceLimitError("gaussian longer than 33 taps")
+        self.gauss_mode = _mode_code(gauss_mode)
+        self.edge_mode = _mode_code(edge_mode)
+        self.gauss_cval = float(gauss_cval)
+        # sobel magnitude weights as the edge filter forms them (float32), as
+        # correlation weights (the convolution weights reversed)
+        edge = (np.array([1, 0, -1], dtype=np.float32) / math.sqrt(2)).astype(np.float32)
+        smooth = (np.array([1, 2, 1]) / 4).astype(np.float32)
+        self.edge3 = np.ascontiguousarray(edge[::-1], dtype=np.float64)
+        self.smooth3 = np.ascontiguousarray(smooth[::-1], dtype=np.float64)
+        self.pixels = sum(int(x.shape[0]) * int(x.shape[1]) for x in self.inputs)
+        self._tables = {}
+        self.rows = rows if rows is not None else _TUNED.get(self.signature(), 16)
+        if self.rows not in self.ROWS:
+            raise ValueError("rows must be 16 or 32")
+
+    def signature(self):
+        shapes = tuple(sorted((int(x.shape[0]), int(x.shape[1])) for x in self.inputs))
+        return ("gray_gauss_sobel", self.taps.w.size, hash(shapes))
+
+    def _table(self, rows):
+        if rows in self._tables:
+            return self._tables[rows]
+        lib = _lib.lib()
+        n = len(self.inputs)
+        tiles = (HCTile2d * n)()
+        for i, (x, o) in enumerate(zip(self.inputs, self.outputs)):
+            tiles[i].in_ = x.data_ptr()
+            tiles[i].out = o.data_ptr()
+            tiles[i].H = int(x.shape[0])
+            tiles[i].W = int(x.shape[1])
+        K = self.taps.w.size
+        total = lib.hc_batch2d_plan(tiles, n, K, rows, None, 0)
+        if total < 0:
+            _lib.check(_lib.ERR_VALUE, "batch plan")
+        item_tile = np.empty(max(total, 1), dtype=np.int32)
+        got = lib.hc_batch2d_plan(tiles, n, K, rows,
+                                  item_tile.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), total)
+        if got != total:
+            _lib.check(_lib.ERR_VALUE, "batch plan")
+        raw = np.frombuffer(bytes(tiles), dtype=np.uint8)
+        dev = _array.device()
+        tab = (torch.from_numpy(raw.copy()).to(dev), torch.from_numpy(item_tile).to(dev), total)
+        self._tables[rows] = tab
+        return tab
+
+    def run(self):
+        """Launch the whole batch on the current stream; returns the outputs."""
+        tiles_dev, items_dev, total = self._table(self.rows)
+        lib = _lib.lib()
+        P = ctypes.POINTER(ctypes.c_double)
+        rc = lib.hc_batch2d_gray_gauss_sobel(
+            tiles_dev.data_ptr(), len(self.inputs), items_dev.data_ptr(), total,
+            self.taps.struct, self.gauss_mode, self.gauss_cval,
+            self.edge3.ctypes.data_as(P), self.smooth3.ctypes.data_as(P), self.edge_mode,
+            self.rows, _array.stream())
+        _lib.check(rc, "batched gray/gaussian/sobel")
+        return self.outputs
+
+    def autotune(self, reps=3):
+        """Time every candidate tile shape on the bound images and keep the
+        fastest (remembered for plans with the same signature)."""
+        best = None
+        for rows in self.ROWS:
+            self.rows = rows
+            ms = _time_ms(self.run, reps)
+            if best is None or ms < best[1]:
+                best = (rows, ms)
+        self.rows = best[0]
+        _TUNED[self.signature()] = self.rows
+        return {"rows": self.rows, "ms": best[1]}
+
+    def capture(self):
+        """A torch CUDA (HIP) graph replaying run(); inputs are read from the
+        bound tensors at each replay."""
+        self._table(self.rows)
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self.run()  # warm-up outside capture
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run()
+        return g

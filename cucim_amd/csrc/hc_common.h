@@ -44,6 +44,28 @@ void timing_end(int slot, hipStream_t st);
     if (_e != hipSuccess) return hc::hip_check(_e, #call); \
   } while (0)
 
+// ------------------------------------------------- separate rounding --
+// Separately rounded multiply / add: the reference forms these in separate
+// kernels (square in place, then accumulate), so they must not contract into
+// a fused multiply-add (the pragma removes the contraction permission from
+// these operations, which survives inlining).
+__device__ __forceinline__ float mul_rn(float a, float b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+__device__ __forceinline__ double mul_rn(double a, double b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+__device__ __forceinline__ float add_rn(float a, float b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+__device__ __forceinline__ double add_rn(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+
 // --------------------------------------------------------------- dtypes --
 inline int dtype_size(int dt) {
   switch (dt) {

@@ -21,6 +21,7 @@
 #include <cstring>
 
 #include "hc_conv.h"
+#include "hc_stencil.h"
 
 namespace hc {
 
@@ -780,6 +781,12 @@ int hc_separable2d(const hc_array* in, const hc_array* out, const hc_taps* taps0
   if (!taps0 || !taps1) return fail(HC_ERR_VALUE, "null taps");
   if (taps0->w_f64 != taps1->w_f64) return fail(HC_ERR_VALUE, "weights dtypes differ");
   hipStream_t st = (hipStream_t)stream;
+  if (taps0->K < 1 || taps1->K < 1) return fail(HC_ERR_VALUE, "incorrect filter size");
+  if (taps0->anchor < 0 || taps0->anchor >= taps0->K || taps1->anchor < 0 ||
+      taps1->anchor >= taps1->K)
+    return fail(HC_ERR_VALUE, "invalid origin");
+  rc = launch_sep2d_reg(in, out, taps0, taps1, mode0, mode1, cval, st);
+  if (rc != -1) return rc;
   return taps0->w_f64 ? launch_sep2d<double>(in, out, taps0, taps1, mode0, mode1, cval, st)
                       : launch_sep2d<float>(in, out, taps0, taps1, mode0, mode1, cval, st);
 }

@@ -36,26 +36,6 @@ bool is_signed_int(int dt) {
 }
 }  // namespace
 
-// Separately rounded multiply / add: the reference forms these in separate
-// kernels (square in place, then accumulate), so they must not contract into
-// a fused multiply-add (the pragma removes the contraction permission from
-// these operations, which survives inlining).
-__device__ __forceinline__ float mul_rn(float a, float b) {
-#pragma clang fp contract(off)
-  return a * b;
-}
-__device__ __forceinline__ double mul_rn(double a, double b) {
-#pragma clang fp contract(off)
-  return a * b;
-}
-__device__ __forceinline__ float add_rn(float a, float b) {
-#pragma clang fp contract(off)
-  return a + b;
-}
-__device__ __forceinline__ double add_rn(double a, double b) {
-#pragma clang fp contract(off)
-  return a + b;
-}
 
 // one tap, skipped when the weight is zero (as the reference skips it)
 template <typename F>

@@ -1,0 +1,382 @@
+// Register-blocked fused 2-D separable filter (float32) for gfx950.
+//
+// Reference behaviour: two calls of the correlate1d line kernel in
+// _run_1d_filters (python/cucim/src/cucim/skimage/_vendored/
+// _ndimage_filters_core.py:124-163), axis 0 then axis 1, float32 weights,
+// float32 intermediate; boundary modes of _vendored/_ndimage_util.py:186-308.
+//
+// k_sep2d_reg<K, TY>: one work-group per TY x TX output tile (TX from
+// reg_strip_outputs(K)); lane c owns staged column x0 - anchor1 + c.
+#include <cstring>
+
+#include "hc_stencil.h"
+
+namespace hc {
+
+struct Reg2dArgs {
+  const float* in;
+  float* out;
+  int64_t H, W;
+  int64_t ntx, ntiles;
+  int TX;
+  int a0, a1;  // anchors of the vertical / horizontal kernel
+  int mode0, mode1;
+  float cval;
+};
+
+template <int K, int TY>
+__global__ __launch_bounds__(256) void k_sep2d_reg(const Reg2dArgs a, const RegTaps<K> t0,
+                                                   const RegTaps<K> t1) {
+  __shared__ __align__(16) float U[TY * kRegUStride];
+  const int64_t tile = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int64_t ty = tile / a.ntx, tx = tile - ty * a.ntx;
+  const int64_t y0 = ty * TY, x0 = tx * a.TX;
+  const int c = threadIdx.x;
+  // staged column; -1: outside the image in constant mode (intermediate = cval)
+  const int64_t xr = remap(x0 - a.a1 + c, a.W, a.mode1);
+  const int64_t ya = y0 - a.a0;
+  float v[TY + K - 1];
+#pragma unroll
+  for (int j = 0; j < TY + K - 1; ++j) {
+    const int64_t yr = remap(ya + j, a.H, a.mode0);
+    v[j] = (yr >= 0 && xr >= 0) ? a.in[yr * a.W + xr] : a.cval;
+  }
+  float acc[TY];
+  reg_vpass<K, TY>(acc, v, t0);
+#pragma unroll
+  for (int i = 0; i < TY; ++i) U[i * kRegUStride + c] = xr < 0 ? a.cval : acc[i];
+  __syncthreads();
+  const int nq = a.TX / kRegP;
+  for (int it = threadIdx.x; it < TY * nq; it += 256) {
+    const int i = it / nq, o = (it - i * nq) * kRegP;
+    const int64_t y = y0 + i, x = x0 + o;
+    if (y >= a.H || x >= a.W) continue;
+    float r[kRegP];
+    reg_hpass<K>(r, U + i * kRegUStride + o, t1);
+    float* op = a.out + y * a.W + x;
+    if (x + kRegP <= a.W && ((uintptr_t)op & 15) == 0) {
+      reinterpret_cast<float4*>(op)[0] = make_float4(r[0], r[1], r[2], r[3]);
+      reinterpret_cast<float4*>(op)[1] = make_float4(r[4], r[5], r[6], r[7]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < kRegP; ++j)
+        if (x + j < a.W) op[j] = r[j];
+    }
+  }
+}
+
+namespace {
+
+template <int K>
+void fill_taps(RegTaps<K>* r, const hc_taps* t) {
+  std::memset(r, 0, sizeof(*r));
+  for (int64_t k = 0; k < t->K; ++k) r->w[k] = (float)t->w[k];
+}
+
+template <int K, int TY>
+int launch_reg(Reg2dArgs a, const hc_taps* t0, const hc_taps* t1, hipStream_t st) {
+  RegTaps<K> r0, r1;
+  fill_taps<K>(&r0, t0);
+  fill_taps<K>(&r1, t1);
+  a.TX = reg_strip_outputs(K);
+  a.ntx = (a.W + a.TX - 1) / a.TX;
+  const int64_t nty = (a.H + TY - 1) / TY;
+  a.ntiles = a.ntx * nty;
+  if (a.ntiles > 0x7fffffff) return fail(HC_ERR_VALUE, "array too large");
+  HC_LAUNCH("k_sep2d_reg", st, (k_sep2d_reg<K, TY>), dim3((unsigned)a.ntiles), dim3(256), 0,
+            a, r0, r1);
+  return HC_OK;
+}
+
+template <int K>
+int launch_reg_ty(const Reg2dArgs& a, const hc_taps* t0, const hc_taps* t1, hipStream_t st) {
+  int ty = tune_get(HC_TUNE_SEP2D_TY);
+  if (ty != 16 && ty != 32) {
+    // the taller tile when it still leaves 4 work-groups per CU
+    const int64_t ntx = (a.W + reg_strip_outputs(K) - 1) / reg_strip_outputs(K);
+    ty = ntx * ((a.H + 31) / 32) >= 4 * 256 ? 32 : 16;
+  }
+  return ty == 32 ? launch_reg<K, 32>(a, t0, t1, st) : launch_reg<K, 16>(a, t0, t1, st);
+}
+
+}  // namespace
+
+int launch_sep2d_reg(const hc_array* in, const hc_array* out, const hc_taps* t0,
+                     const hc_taps* t1, int mode0, int mode1, double cval, hipStream_t st) {
+  if (in->dtype != HC_FLOAT32 || out->dtype != HC_FLOAT32 || t0->w_f64 || t1->w_f64) return -1;
+  if (!is_c_contiguous(in) || !is_c_contiguous(out) || in->ndim != 2) return -1;
+  const int64_t K = t0->K > t1->K ? t0->K : t1->K;
+  if (K > kRegMaxK) return -1;
+  Reg2dArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = (const float*)in->data;
+  a.out = (float*)out->data;
+  a.H = in->shape[0];
+  a.W = in->shape[1];
+  a.a0 = t0->anchor;
+  a.a1 = t1->anchor;
+  a.mode0 = mode0;
+  a.mode1 = mode1;
+  a.cval = (float)cval;
+  if (a.H == 0 || a.W == 0) return HC_OK;
+  if (K <= 3) return launch_reg_ty<3>(a, t0, t1, st);
+  if (K <= 5) return launch_reg_ty<5>(a, t0, t1, st);
+  if (K <= 7) return launch_reg_ty<7>(a, t0, t1, st);
+  if (K <= 9) return launch_reg_ty<9>(a, t0, t1, st);
+  if (K <= 13) return launch_reg_ty<13>(a, t0, t1, st);
+  if (K <= 17) return launch_reg_ty<17>(a, t0, t1, st);
+  if (K <= 21) return launch_reg_ty<21>(a, t0, t1, st);
+  if (K <= 25) return launch_reg_ty<25>(a, t0, t1, st);
+  return launch_reg_ty<33>(a, t0, t1, st);
+}
+
+// ------------------------------------------------------------------------
+// Batched rgb2gray -> gaussian -> edge magnitude over uneven tiles.
+//
+// Reference behaviour, per tile (python/cucim/src/cucim/skimage/):
+//   color/colorconv.py:1037-1095   gray = 0.2125 R + 0.7154 G + 0.0721 B of
+//                                   img_as_float(rgb) (float32 for uint8),
+//                                   summed in double, stored in float32
+//   _shared/filters.py:21-166       gaussian: axis 0 then axis 1 passes
+//   filters/edges.py:148-241        edge magnitude: edge pass then smoothing
+//                                   pass per axis, sqrt(A*A + B*B)
+// Every stage rounds to float32 exactly where the separate calls store
+// their float32 outputs, so the fused result equals the three calls'.
+//
+// One work item = TY x TX output pixels of one tile.  The gaussian output G
+// is formed with a one-pixel halo (TY + 2 rows, GX = TX + 2 columns) so the
+// 3x3 edge stencils read it from LDS; halo rows / columns outside the image
+// are replaced by the edge filter's boundary values (reflect, mirror,
+// nearest: the remapped G value, which lies inside the item; constant: 0,
+// as the reference does not forward cval to the edge filter).
+namespace {
+
+struct GgsArgs {
+  const hc_tile2d* tiles;
+  const int32_t* item_tile;
+  int64_t n_items;
+  int TX;      // output columns per item
+  int ag;      // gaussian anchor
+  int gmode;   // gaussian boundary mode
+  int emode;   // edge-filter boundary mode (not wrap)
+  float gcval;
+  float scale;  // float32(1 / 255)
+  float e[3], s[3];  // edge / smoothing correlation weights
+};
+
+__device__ __forceinline__ float gray_u8(const uint8_t* __restrict__ p, float scale) {
+  const double r = (double)((float)p[0] * scale);
+  const double g = (double)((float)p[1] * scale);
+  const double b = (double)((float)p[2] * scale);
+  return (float)add_rn(add_rn(mul_rn(0.2125, r), mul_rn(0.7154, g)), mul_rn(0.0721, b));
+}
+
+// one tap with the reference's zero-weight skip
+__device__ __forceinline__ float etap(float acc, float x, float w) {
+  return wnz(w) ? __builtin_fmaf(x, w, acc) : acc;
+}
+
+}  // namespace
+
+template <int K, int TY>
+__global__ __launch_bounds__(256) void k_tiles_ggs(const GgsArgs a, const RegTaps<K> tg) {
+  constexpr int GR = TY + 2;  // gaussian rows with the edge halo
+  __shared__ __align__(16) float V[GR * kRegUStride];
+  __shared__ __align__(16) float G[GR * kRegUStride];
+  const int GX = a.TX + 2;
+  const int c = threadIdx.x;
+  const int64_t G_ = gridDim.x;
+  const int64_t slot = xcd_swizzle(blockIdx.x, G_);
+  for (int64_t base = 0; base < a.n_items; base += G_) {
+    const int64_t it = base + slot;
+    if (it >= a.n_items) break;  // uniform per work-group
+    const hc_tile2d d = a.tiles[a.item_tile[it]];
+    const int64_t H = d.H, W = d.W;
+    const int64_t local = it - d.item0;
+    const int64_t nbx = (W + a.TX - 1) / a.TX;
+    const int64_t by = local / nbx, bx = local - by * nbx;
+    const int64_t y0 = by * TY, x0 = bx * a.TX;
+    const uint8_t* in = reinterpret_cast<const uint8_t*>(d.in);
+    // 1. gray + vertical gaussian pass, one staged column per lane
+    {
+      const int64_t xr = remap(x0 - 1 - a.ag + c, W, a.gmode);
+      const int64_t ya = y0 - 1 - a.ag;
+      float v[GR + K - 1];
+#pragma unroll
+      for (int j = 0; j < GR + K - 1; ++j) {
+        const int64_t yr = remap(ya + j, H, a.gmode);
+        v[j] = (yr >= 0 && xr >= 0) ? gray_u8(in + 3 * (yr * W + xr), a.scale) : a.gcval;
+      }
+      float acc[GR];
+      reg_vpass<K, GR>(acc, v, tg);
+#pragma unroll
+      for (int i = 0; i < GR; ++i) V[i * kRegUStride + c] = xr < 0 ? a.gcval : acc[i];
+    }
+    __syncthreads();
+    // 2. horizontal gaussian pass into G
+    {
+      const int nq = GX / kRegP;
+      for (int q = threadIdx.x; q < GR * nq; q += 256) {
+        const int i = q / nq, o = (q - i * nq) * kRegP;
+        float r[kRegP];
+        reg_hpass<K>(r, V + i * kRegUStride + o, tg);
+#pragma unroll
+        for (int j = 0; j < kRegP; ++j) G[i * kRegUStride + o + j] = r[j];
+      }
+    }
+    __syncthreads();
+    // 3. edge-filter boundary of G (items touching the image border only)
+    const bool top = y0 == 0, bottom = y0 + TY + 1 > H, left = x0 == 0,
+               right = x0 + a.TX + 1 > W;
+    if (top || bottom) {
+      const bool cst = a.emode == HC_MODE_CONSTANT;
+      for (int e = threadIdx.x; e < 2 * GX; e += 256) {
+        const int which = e / GX, gc = e - which * GX;
+        const int64_t y = which == 0 ? y0 - 1 : H;  // the halo row to replace
+        const int64_t gr = y - (y0 - 1);
+        if ((which == 0 && !top) || (which == 1 && (!bottom || gr > TY + 1))) continue;
+        float val = 0.f;
+        if (!cst) val = G[(remap(y, H, a.emode) - (y0 - 1)) * kRegUStride + gc];
+        G[gr * kRegUStride + gc] = val;
+      }
+      __syncthreads();
+    }
+    if (left || right) {
+      const bool cst = a.emode == HC_MODE_CONSTANT;
+      for (int e = threadIdx.x; e < 2 * GR; e += 256) {
+        const int which = e / GR, gr = e - which * GR;
+        const int64_t x = which == 0 ? x0 - 1 : W;
+        const int64_t gc = x - (x0 - 1);
+        if ((which == 0 && !left) || (which == 1 && (!right || gc > GX - 1))) continue;
+        float val = 0.f;
+        if (!cst) val = G[gr * kRegUStride + (remap(x, W, a.emode) - (x0 - 1))];
+        G[gr * kRegUStride + gc] = val;
+      }
+      __syncthreads();
+    }
+    // 4. edge magnitude, one output column per lane
+    if (c < a.TX && x0 + c < W) {
+      float* out = reinterpret_cast<float*>(d.out);
+      for (int i = 0; i < TY; ++i) {
+        const int64_t y = y0 + i;
+        if (y >= H) break;
+        const float* g0 = G + i * kRegUStride + c;  // rows y-1..y+1, cols x-1..x+1
+        float ey[3], ex[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          float v = 0.f;
+          v = etap(v, g0[q], a.e[0]);
+          v = etap(v, g0[kRegUStride + q], a.e[1]);
+          v = etap(v, g0[2 * kRegUStride + q], a.e[2]);
+          ey[q] = v;  // edge along axis 0 at column x - 1 + q
+          float u = 0.f;
+          u = etap(u, g0[q * kRegUStride], a.e[0]);
+          u = etap(u, g0[q * kRegUStride + 1], a.e[1]);
+          u = etap(u, g0[q * kRegUStride + 2], a.e[2]);
+          ex[q] = u;  // edge along axis 1 at row y - 1 + q
+        }
+        float A = 0.f, B = 0.f;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          A = etap(A, ey[q], a.s[q]);
+          B = etap(B, ex[q], a.s[q]);
+        }
+        out[y * W + x0 + c] = sqrtf(add_rn(mul_rn(A, A), mul_rn(B, B)));
+      }
+    }
+    __syncthreads();  // V and G are reused by the next item
+  }
+}
+
+namespace {
+
+template <int K, int TY>
+int launch_ggs(GgsArgs a, const hc_taps* g, hipStream_t st) {
+  RegTaps<K> t;
+  fill_taps<K>(&t, g);
+  a.TX = reg_strip_outputs(K) - 2;
+  const int64_t grid = a.n_items < 256 * 8 ? a.n_items : 256 * 8;
+  HC_LAUNCH("k_tiles_ggs", st, (k_tiles_ggs<K, TY>), dim3((unsigned)grid), dim3(256), 0, a, t);
+  return HC_OK;
+}
+
+template <int TY>
+int launch_ggs_k(const GgsArgs& a, const hc_taps* g, hipStream_t st) {
+  const int64_t K = g->K;
+  if (K <= 5) return launch_ggs<5, TY>(a, g, st);
+  if (K <= 9) return launch_ggs<9, TY>(a, g, st);
+  if (K <= 13) return launch_ggs<13, TY>(a, g, st);
+  if (K <= 17) return launch_ggs<17, TY>(a, g, st);
+  if (K <= 25) return launch_ggs<25, TY>(a, g, st);
+  return launch_ggs<33, TY>(a, g, st);
+}
+
+int ggs_k_compiled(int64_t K) {
+  return K <= 5 ? 5 : K <= 9 ? 9 : K <= 13 ? 13 : K <= 17 ? 17 : K <= 25 ? 25 : 33;
+}
+
+}  // namespace
+}  // namespace hc
+
+using namespace hc;
+
+extern "C" {
+
+int64_t hc_batch2d_plan(hc_tile2d* tiles, int n, int K, int ty, int32_t* item_tile,
+                        int64_t cap) {
+  if (!tiles || n < 0) return fail(HC_ERR_VALUE, "null tiles"), -1;
+  if (K < 1 || K > kRegMaxK) return fail(HC_ERR_VALUE, "kernel length outside 1..33"), -1;
+  if (ty != 16 && ty != 32) return fail(HC_ERR_VALUE, "tile rows must be 16 or 32"), -1;
+  const int64_t TX = reg_strip_outputs(ggs_k_compiled(K)) - 2;
+  int64_t total = 0;
+  for (int t = 0; t < n; ++t) {
+    if (tiles[t].H < 0 || tiles[t].W < 0) return fail(HC_ERR_VALUE, "negative tile shape"), -1;
+    tiles[t].item0 = total;
+    const int64_t items = ((tiles[t].H + ty - 1) / ty) * ((tiles[t].W + TX - 1) / TX);
+    if (item_tile) {
+      if (total + items > cap) return fail(HC_ERR_VALUE, "item table too small"), -1;
+      for (int64_t i = 0; i < items; ++i) item_tile[total + i] = t;
+    }
+    total += items;
+  }
+  if (total > 0x7fffffff) return fail(HC_ERR_VALUE, "too many work items"), -1;
+  return total;
+}
+
+int hc_batch2d_gray_gauss_sobel(const hc_tile2d* tiles_dev, int n, const int32_t* item_tile_dev,
+                                int64_t n_items, const hc_taps* gauss, int gauss_mode,
+                                double gauss_cval, const double* edge3, const double* smooth3,
+                                int edge_mode, int ty, void* stream) {
+  if (!tiles_dev || !item_tile_dev || !gauss || !edge3 || !smooth3)
+    return fail(HC_ERR_VALUE, "null argument");
+  if (n < 0 || n_items < 0) return fail(HC_ERR_VALUE, "negative count");
+  if (gauss->w_f64) return fail(HC_ERR_VALUE, "gaussian weights must be float32");
+  if (gauss->K < 1 || gauss->K > kRegMaxK)
+    return fail(HC_ERR_RESOURCE, "gaussian kernel longer than 33 taps");
+  if (gauss->anchor < 0 || gauss->anchor >= gauss->K) return fail(HC_ERR_VALUE, "invalid origin");
+  if (gauss_mode < HC_MODE_REFLECT || gauss_mode > HC_MODE_CONSTANT ||
+      edge_mode < HC_MODE_REFLECT || edge_mode > HC_MODE_CONSTANT)
+    return fail(HC_ERR_VALUE, "boundary mode not supported");
+  if (edge_mode == HC_MODE_WRAP)
+    return fail(HC_ERR_UNSUPPORTED, "wrap mode edges are not fused (halo outside the item)");
+  if (ty != 16 && ty != 32) return fail(HC_ERR_VALUE, "tile rows must be 16 or 32");
+  if (n_items == 0) return HC_OK;
+  GgsArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.tiles = tiles_dev;
+  a.item_tile = item_tile_dev;
+  a.n_items = n_items;
+  a.ag = gauss->anchor;
+  a.gmode = gauss_mode;
+  a.emode = edge_mode;
+  a.gcval = (float)gauss_cval;
+  a.scale = (float)(1.0 / 255.0);
+  for (int k = 0; k < 3; ++k) {
+    a.e[k] = (float)edge3[k];
+    a.s[k] = (float)smooth3[k];
+  }
+  hipStream_t st = (hipStream_t)stream;
+  return ty == 32 ? launch_ggs_k<32>(a, gauss, st) : launch_ggs_k<16>(a, gauss, st);
+}
+
+}  // extern "C"

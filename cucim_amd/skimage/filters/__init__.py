@@ -2,4 +2,4 @@
 
 from .._shared.filters import gaussian  # noqa: F401
 from .edges import farid, prewitt, scharr, sobel  # noqa: F401
-from .ridges import frangi  # noqa: F401
+from .ridges import frangi, hessian, meijering, sato  # noqa: F401

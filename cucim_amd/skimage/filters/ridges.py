@@ -1,4 +1,5 @@
-"""Frangi vesselness (``filters/ridges.py:322-476`` of the reference, paths
+"""Ridge filters: Frangi vesselness, Sato tubeness, Meijering neuriteness
+and the hybrid Hessian filter (``filters/ridges.py`` of the reference, paths
 relative to ``python/cucim/src/cucim/skimage/``)."""
 
 from warnings import warn
@@ -11,12 +12,13 @@ from .._shared.utils import _supported_float_type, astype, check_nD
 from .._vendored._ndimage_util import _mode_code
 from ..feature.corner import (
     _hessian_sequential,
+    hessian_matrix,
     _symmetric_compute_eigenvalues,
     _taps_arrays,
     hessian_kernels,
 )
 
-__all__ = ["frangi"]
+__all__ = ["frangi", "hessian", "meijering", "sato"]
 
 
 class FrangiPlan:
@@ -103,3 +105,69 @@ def frangi(image, sigmas=range(1, 10, 2), scale_range=None, scale_step=None, alp
         _lib.check(rc, "frangi")
     del plan
     return _frangi_sequential(image, sigmas, alpha, beta, gamma, black_ridges, mode, cval, out)
+
+
+def _ridge_input(image, black_ridges):
+    image = _array.as_tensor(image)
+    float_dtype = _supported_float_type(image.dtype)
+    image = astype(image, float_dtype)
+    if not black_ridges:
+        image = -image
+    return image, float_dtype
+
+
+def sato(image, sigmas=range(1, 10, 2), black_ridges=True, mode="reflect", cval=0):
+    """Sato tubeness filter (``ridges.py:148-225``): per sigma, the geometric
+    mean of the positive Hessian eigenvalues other than the smallest, times
+    sigma^2; maximum over sigmas (from 0)."""
+    check_nD(_array.as_tensor(image), [2, 3])
+    image, float_dtype = _ridge_input(image, black_ridges)
+    out = _array.empty(tuple(image.shape), float_dtype)
+    lib = _lib.lib()
+    for i, sigma in enumerate(sigmas):
+        H = hessian_matrix(image, sigma, mode=mode, cval=cval, use_gaussian_derivatives=True)
+        _lib.check(lib.hc_sato_fold(_array.hc_arrays(H), len(H), _array.hc_array(out),
+                                    float(sigma), 1 if i == 0 else 0, _array.stream()), "sato")
+    if len(list(sigmas)) == 0:
+        out.zero_()
+    return out
+
+
+def meijering(image, sigmas=range(1, 10, 2), alpha=None, black_ridges=True, mode="reflect",
+              cval=0):
+    """Meijering neuriteness filter (``ridges.py:50-145``): per sigma, the
+    normalised eigenvalue combination of largest magnitude, clipped at 0 and
+    scaled by its image maximum; maximum over sigmas (from 0).  2-D and 3-D
+    images (the closed-form eigenvalues)."""
+    image, float_dtype = _ridge_input(image, black_ridges)
+    if image.dim() not in (2, 3):
+        raise NotImplementedError("meijering is implemented for 2-D and 3-D images")
+    if alpha is None:
+        alpha = 1 / (image.dim() + 1)
+    out = _array.empty(tuple(image.shape), float_dtype)
+    tmp = _array.empty(tuple(image.shape), float_dtype)
+    scratch = torch.zeros(64, dtype=torch.uint8, device=_array.device())
+    lib = _lib.lib()
+    n = 0
+    for i, sigma in enumerate(sigmas):
+        H = hessian_matrix(image, sigma, mode=mode, cval=cval, use_gaussian_derivatives=True)
+        _lib.check(lib.hc_meijering_fold(_array.hc_arrays(H), len(H), _array.hc_array(out),
+                                         _array.hc_array(tmp), float(alpha),
+                                         1 if i == 0 else 0, scratch.data_ptr(),
+                                         _array.stream()), "meijering")
+        n += 1
+    if n == 0:
+        out.zero_()
+    return out
+
+
+def hessian(image, sigmas=range(1, 10, 2), scale_range=None, scale_step=None, alpha=0.5,
+            beta=0.5, gamma=15, black_ridges=True, mode="reflect", cval=0):
+    """Hybrid Hessian filter (``ridges.py:479-565``): Frangi with gamma = 15
+    by default, then every value <= 0 set to 1."""
+    filtered = frangi(image, sigmas=sigmas, scale_range=scale_range, scale_step=scale_step,
+                      alpha=alpha, beta=beta, gamma=gamma, black_ridges=black_ridges,
+                      mode=mode, cval=cval)
+    _lib.check(_lib.lib().hc_fill_le(_array.hc_array(filtered), 0.0, 1.0, _array.stream()),
+               "hessian")
+    return filtered

@@ -111,6 +111,39 @@ typedef enum hc_tune_knob {
 int hc_tune_set(int knob, int value);
 int hc_tune_get(int knob);
 
+/* One image of a batched 2-D pipeline (no counterpart in the reference,
+ * which filters one image per call).  item0 is filled by hc_batch2d_plan. */
+typedef struct hc_tile2d {
+  const void* in;   /* input image, C-contiguous */
+  void* out;        /* output image, C-contiguous */
+  int64_t H, W;     /* rows, columns */
+  int64_t item0;    /* index of the image's first work item */
+} hc_tile2d;
+
+/* Host-side plan of a batched 2-D pipeline over n images: fills
+ * tiles[i].item0 and, if item_tile is not NULL, item_tile[0 .. total) with
+ * the image of each work item (cap entries available).  K: gaussian kernel
+ * length, ty: output rows per work item (16 or 32).  Returns the number of
+ * work items, or -1 (message in hc_last_error()). */
+int64_t hc_batch2d_plan(hc_tile2d* tiles, int n, int K, int ty, int32_t* item_tile,
+                        int64_t cap);
+
+/* rgb2gray -> gaussian -> edge magnitude of many uneven uint8 RGB images in
+ * one persistent launch, each output equal to
+ *   filters.sobel(filters.gaussian(color.rgb2gray(rgb), sigma, mode=gauss_mode,
+ *                                  cval=gauss_cval), mode=edge_mode)
+ * (color/colorconv.py:1037-1095, _shared/filters.py:21-166,
+ * filters/edges.py:148-241; the edge filter's weights given as correlation
+ * weights, edge already divided by sqrt(2)).  tiles_dev / item_tile_dev:
+ * device copies of the planned table (inputs (H, W, 3) uint8, outputs
+ * (H, W) float32).  Gaussian taps float32, K <= 33.  edge_mode 'wrap' is
+ * HC_ERR_UNSUPPORTED (run the three calls instead). */
+int hc_batch2d_gray_gauss_sobel(const hc_tile2d* tiles_dev, int n,
+                                const int32_t* item_tile_dev, int64_t n_items,
+                                const hc_taps* gauss, int gauss_mode, double gauss_cval,
+                                const double* edge3, const double* smooth3, int edge_mode,
+                                int ty, void* stream);
+
 /* correlate1d / convolve1d core: _vendored/_ndimage_filters.py:271-344
  * (_correlate_or_convolve1d), the shared-memory kernel of
  * filters/_separable_filtering.py:977-1070 and the elementwise kernel of
@@ -176,6 +209,25 @@ int hc_frangi(const hc_array* in, const hc_array* out, const hc_taps* taps_s,
 int hc_frangi_from_eigvals(const hc_array* eig, int ndim, const hc_array* out,
                            double alpha, double beta, double gamma,
                            int first_sigma, void* scratch, void* stream);
+
+/* Sato tubeness of one sigma folded into the running maximum over sigmas:
+ * filters/ridges.py:148-225 after its hessian_matrix call.  H: the
+ * ndim*(ndim+1)/2 Hessian elements (combinations_with_replacement order),
+ * out: running maximum (first_sigma != 0 starts it from 0). */
+int hc_sato_fold(const hc_array* H, int nH, const hc_array* out, double sigma,
+                 int first_sigma, void* stream);
+
+/* Meijering neuriteness of one sigma folded into the running maximum:
+ * filters/ridges.py:50-145 after its hessian_matrix call (normalised by
+ * the image maximum of the sigma's response, reduced on the device; no host
+ * synchronisation).  tmp: an array like out; scratch: >= 8 bytes of device
+ * memory. */
+int hc_meijering_fold(const hc_array* H, int nH, const hc_array* out, const hc_array* tmp,
+                      double alpha, int first_sigma, void* scratch, void* stream);
+
+/* a[a <= threshold] = value (float32 / float64): the last step of the hybrid
+ * Hessian filter, filters/ridges.py:479-565. */
+int hc_fill_le(const hc_array* a, double threshold, double value, void* stream);
 
 /* rgb2gray: color/colorconv.py:1037-1095 (+ _prepare_colorarray :160-180).
  * in: (..., 3) C-contiguous, any real dtype; out: (...) float32/float64. */

@@ -16,6 +16,8 @@ Reference files (relative to ``python/cucim/src/cucim/skimage/``):
   color/colorconv.py:160-180,1037-1095  rgb2gray
   feature/corner.py:161-856        hessian_matrix, eigenvalue kernels
   filters/ridges.py:228-476        frangi
+  filters/ridges.py:50-225         meijering, sato
+  filters/ridges.py:479-565        hessian (hybrid Hessian filter)
   _shared/_gradient.py             finite-difference gradient
 """
 
@@ -43,6 +45,9 @@ __all__ = [
     "symmetric_compute_eigenvalues",
     "hessian_matrix_eigvals",
     "frangi",
+    "sato",
+    "meijering",
+    "hessian",
 ]
 
 
@@ -480,3 +485,76 @@ def frangi(image, sigmas=range(1, 10, 2), scale_range=None, scale_step=None,
     if return_gamma:
         return filtered_max, gamma_sq
     return filtered_max
+
+
+def _ridge_eigvals(image, sigma, mode, cval, ndi, F):
+    """hessian_matrix (Gaussian derivatives) + hessian_matrix_eigvals
+    (descending), eigenvalues below 1e-12 in magnitude zeroed
+    (``ridges.py:123-133, 205-215``)."""
+    H = hessian_matrix(image, sigma, mode=mode, cval=cval, use_gaussian_derivatives=True,
+                       ndi=ndi, dtype=F)
+    ev = hessian_matrix_eigvals([np.asarray(h).astype(F) for h in H]).astype(F)
+    ev[np.abs(ev) < 1e-12] = 0
+    return ev
+
+
+def sato(image, sigmas=range(1, 10, 2), black_ridges=True, mode="reflect", cval=0, *,
+         ndi=None, dtype=None):
+    """``filters/ridges.py:148-225``: per sigma, the geometric mean of the
+    eigenvalues other than the smallest, each clipped at 0, times sigma^2;
+    pixel-wise maximum over sigmas, starting from 0."""
+    image = np.asarray(image)
+    if image.ndim not in (2, 3):
+        raise ValueError("The parameter `image` must be a 2-dimensional or 3-dimensional array")
+    F = supported_float_type(image.dtype) if dtype is None else np.dtype(dtype)
+    image = image.astype(F, copy=False)
+    if not black_ridges:
+        image = -image
+    out = np.zeros(image.shape, F)
+    for sigma in sigmas:
+        ev = _ridge_eigvals(image, sigma, mode, cval, ndi, F)[:-1]
+        with np.errstate(invalid="ignore"):
+            vals = np.prod(np.maximum(ev, 0), 0) ** (1 / len(ev))
+        vals = (vals * F.type(sigma ** 2)).astype(F)
+        out = np.maximum(out, vals)
+    return out
+
+
+def meijering(image, sigmas=range(1, 10, 2), alpha=None, black_ridges=True, mode="reflect",
+              cval=0, *, ndi=None, dtype=None):
+    """``filters/ridges.py:50-145``: normalised eigenvalues
+    l_i = e_i + alpha * sum_{j != i} e_j, the one of largest magnitude,
+    clipped at 0, divided by its image maximum (when positive); pixel-wise
+    maximum over sigmas, starting from 0."""
+    image = np.asarray(image)
+    F = supported_float_type(image.dtype) if dtype is None else np.dtype(dtype)
+    image = image.astype(F, copy=False)
+    if not black_ridges:
+        image = -image
+    if alpha is None:
+        alpha = 1 / (image.ndim + 1)
+    mtx = np.full((image.ndim, image.ndim), alpha, dtype=F)
+    np.fill_diagonal(mtx, 1)
+    out = np.zeros(image.shape, F)
+    for sigma in sigmas:
+        ev = _ridge_eigvals(image, sigma, mode, cval, ndi, F)
+        vals = np.tensordot(mtx, ev, 1)
+        vals = np.take_along_axis(vals, np.abs(vals).argmax(0)[None], 0).squeeze(0)
+        vals = np.maximum(vals, 0)
+        mx = vals.max()
+        if mx > 0:
+            vals = vals / mx
+        out = np.maximum(out, vals.astype(F))
+    return out
+
+
+def hessian(image, sigmas=range(1, 10, 2), scale_range=None, scale_step=None, alpha=0.5,
+            beta=0.5, gamma=15, black_ridges=True, mode="reflect", cval=0, *, ndi=None,
+            dtype=None):
+    """``filters/ridges.py:479-565``: frangi (gamma 15 by default), then
+    every value <= 0 set to 1."""
+    out = frangi(image, sigmas=sigmas, scale_range=scale_range, scale_step=scale_step,
+                 alpha=alpha, beta=beta, gamma=gamma, black_ridges=black_ridges, mode=mode,
+                 cval=cval, ndi=ndi, dtype=dtype)
+    out[out <= 0] = 1
+    return out

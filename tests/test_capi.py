@@ -90,3 +90,27 @@ def test_no_unresolved_library_symbols():
                          text=True, check=True).stdout
     bad = [ln for ln in out.splitlines() if "_ZN2hc" in ln or " hc_" in ln]
     assert not bad, bad
+
+
+def test_batch2d_plan_counts():
+    """Work-item planning is host-only: items = ceil(H/ty) * ceil(W/TX) per
+    image, TX = 238 output columns for a 13-tap gaussian."""
+    from cucim_amd.batch import HCTile2d
+
+    lib = _lib.load_library()
+    shapes = [(256, 256), (1, 1), (0, 5), (300, 239), (17, 1000)]
+    tiles = (HCTile2d * len(shapes))()
+    for i, (h, w) in enumerate(shapes):
+        tiles[i].H, tiles[i].W = h, w
+    for ty in (16, 32):
+        exp = [(-(-h // ty)) * (-(-w // 238)) for h, w in shapes]
+        total = lib.hc_batch2d_plan(tiles, len(shapes), 13, ty, None, 0)
+        assert total == sum(exp)
+        table = (ctypes.c_int32 * total)()
+        assert lib.hc_batch2d_plan(tiles, len(shapes), 13, ty, table, total) == total
+        assert list(table) == [i for i, n in enumerate(exp) for _ in range(n)]
+        assert [tiles[i].item0 for i in range(len(shapes))] == list(np.cumsum([0] + exp[:-1]))
+    assert lib.hc_batch2d_plan(tiles, len(shapes), 13, 24, None, 0) == -1
+    assert lib.hc_batch2d_plan(tiles, len(shapes), 40, 16, None, 0) == -1
+    small = (ctypes.c_int32 * 3)()
+    assert lib.hc_batch2d_plan(tiles, len(shapes), 13, 16, small, 3) == -1

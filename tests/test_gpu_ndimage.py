@@ -187,7 +187,7 @@ def test_noncontiguous_input_and_output(gpu):
 ])
 @pytest.mark.parametrize("mode", ["nearest", "reflect", "constant", "wrap", "mirror"])
 @pytest.mark.parametrize("shape", [(64, 57), (300, 200), (7, 9, 11)])
-@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("fused", [False, True, None])
 def test_separable_filters(gpu, fn, kw, mode, shape, fused, monkeypatch):
     from cucim_amd.skimage._vendored import _ndimage_filters as F
 
@@ -280,7 +280,7 @@ def test_non_finite_inputs(gpu, shape, fn, kw):
         try:
             _same_with_nan(_run(getattr(ndi, fn), x, mode="reflect", **kw), exp)
         finally:
-            F.FUSED_2D = False
+            F.FUSED_2D = None
 
 
 @pytest.mark.parametrize("seg", [256, 512, 2048, 4096])
@@ -302,7 +302,9 @@ def test_tuned_contig_segment(gpu, seg):
 
 @pytest.mark.parametrize("ty", [16, 32, 64])
 @pytest.mark.parametrize("shape", [(300, 260), (257, 131)])
-def test_tuned_sep2d_rows(gpu, ty, shape, monkeypatch):
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_tuned_sep2d_rows(gpu, ty, shape, dtype, monkeypatch):
+    """float32: the register-blocked kernel; float64: the LDS tile kernel."""
     from cucim_amd import _lib
     from cucim_amd.skimage._vendored import _ndimage_filters as F
 
@@ -310,7 +312,7 @@ def test_tuned_sep2d_rows(gpu, ty, shape, monkeypatch):
 
     ndi = _ndi()
     lib = _lib.lib()
-    x = _img(shape, np.float32)
+    x = _img(shape, dtype)
     exp = on.gaussian_filter(x, 2.0, mode="constant", cval=0.5, acc="emulate")
     _lib.check(lib.hc_tune_set(_lib.TUNE_SEP2D_TY, ty))
     try:
@@ -327,3 +329,29 @@ def test_tuning_knob_errors(gpu):
     assert lib.hc_tune_set(_lib.TUNE_SEP2D_TY, 24) == _lib.ERR_VALUE
     assert lib.hc_tune_set(99, 1) == _lib.ERR_VALUE
     assert lib.hc_tune_get(_lib.TUNE_CONTIG_SEG) == 0
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 300), (300, 1), (5, 7), (17, 250), (33, 241),
+                                   (64, 480), (100, 1001)])
+@pytest.mark.parametrize("sigma", [(1.0, 3.0), (4.0, 0.5), 2.0, (7.9, 7.9)])
+@pytest.mark.parametrize("mode", MODES)
+def test_register_blocked_2d(gpu, shape, sigma, mode, monkeypatch):
+    """The register-blocked fused kernel: uneven kernel lengths (zero-padded
+    to a compiled length), all modes, shapes smaller than a tile."""
+    from cucim_amd.skimage._vendored import _ndimage_filters as F
+
+    monkeypatch.setattr(F, "FUSED_2D", None)
+    ndi = _ndi()
+    x = _img(shape, np.float32, seed=5)
+    exp = on.gaussian_filter(x, sigma, mode=mode, cval=-0.75, acc="emulate")
+    _exact(_run(ndi.gaussian_filter, x, sigma, mode=mode, cval=-0.75), exp)
+
+
+@pytest.mark.parametrize("size,origin", [((3, 9), 0), ((9, 3), (1, -4)), ((1, 5), (0, 2)),
+                                         ((4, 6), (-2, 2))])
+@pytest.mark.parametrize("mode", ["wrap", "constant", "mirror"])
+def test_register_blocked_uniform(gpu, size, origin, mode):
+    ndi = _ndi()
+    x = _img((70, 90), np.float32, seed=6)
+    exp = on.uniform_filter(x, size, mode=mode, cval=2.0, origin=origin, acc="emulate")
+    _exact(_run(ndi.uniform_filter, x, size, mode=mode, cval=2.0, origin=origin), exp)
