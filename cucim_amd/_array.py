@@ -24,6 +24,7 @@ _TORCH_TO_HC = {
     torch.float16: _lib.FLOAT16,
     torch.float32: _lib.FLOAT32,
     torch.float64: _lib.FLOAT64,
+    torch.bfloat16: _lib.BFLOAT16,
 }
 
 _NP_TO_TORCH = {

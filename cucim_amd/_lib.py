@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(_HERE, "lib", "libcucim_amd.so")
 MAX_NDIM = 8
 
 # hc_dtype
-BOOL, INT8, UINT8, INT16, UINT16, INT32, UINT32, INT64, UINT64, FLOAT16, FLOAT32, FLOAT64 = range(12)
+(BOOL, INT8, UINT8, INT16, UINT16, INT32, UINT32, INT64, UINT64, FLOAT16, FLOAT32, FLOAT64,
+ BFLOAT16) = range(13)
 
 # hc_mode
 MODE_REFLECT, MODE_MIRROR, MODE_NEAREST, MODE_WRAP, MODE_CONSTANT = range(5)
@@ -60,6 +61,15 @@ ENTRY_POINTS = (
     "hc_sato_fold",
     "hc_meijering_fold",
     "hc_fill_le",
+    "hc_cnn3d_packed_size",
+    "hc_cnn3d_pack",
+    "hc_cnn3d_conv",
+    "hc_cnn3d_pad",
+    "hc_cnn3d_gelu_bwd",
+    "hc_cnn3d_dgrad",
+    "hc_cnn3d_wgrad",
+    "hc_cnn3d_mse_grad",
+    "hc_cnn3d_adam",
 )
 
 
@@ -127,6 +137,15 @@ _SIGNATURES = {
     "hc_sato_fold": ([_A, _i, _A, _d, _i, _v], _i),
     "hc_meijering_fold": ([_A, _i, _A, _A, _d, _i, _v, _v], _i),
     "hc_fill_le": ([_A, _d, _d, _v], _i),
+    "hc_cnn3d_packed_size": ([_i, _i], _i),
+    "hc_cnn3d_pack": ([_v, _v, _i, _i, _v], _i),
+    "hc_cnn3d_conv": ([_A, _A, _v, _v, _i, _v, _v], _i),
+    "hc_cnn3d_pad": ([_A, _A, _i, _i, _d, _v], _i),
+    "hc_cnn3d_gelu_bwd": ([_v, _v, ctypes.c_int64, _v], _i),
+    "hc_cnn3d_dgrad": ([_v, _v, _v, _i, _i, _i, _i, _i, _v], _i),
+    "hc_cnn3d_wgrad": ([_v, _v, _v, _v, _i, _i, _i, _i, _i, _v], _i),
+    "hc_cnn3d_mse_grad": ([_v, _v, _v, ctypes.c_int64, _v, _v], _i),
+    "hc_cnn3d_adam": ([_v, _v, _v, _v, ctypes.c_int64, _d, _d, _d, _d, _i, _v], _i),
 }
 
 _lock = threading.Lock()

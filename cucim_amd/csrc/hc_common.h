@@ -70,7 +70,7 @@ __device__ __forceinline__ double add_rn(double a, double b) {
 inline int dtype_size(int dt) {
   switch (dt) {
     case HC_BOOL: case HC_INT8: case HC_UINT8: return 1;
-    case HC_INT16: case HC_UINT16: case HC_FLOAT16: return 2;
+    case HC_INT16: case HC_UINT16: case HC_FLOAT16: case HC_BFLOAT16: return 2;
     case HC_INT32: case HC_UINT32: case HC_FLOAT32: return 4;
     case HC_INT64: case HC_UINT64: case HC_FLOAT64: return 8;
     default: return 0;

@@ -37,7 +37,8 @@ typedef enum hc_dtype {
   HC_UINT64 = 8,
   HC_FLOAT16 = 9,
   HC_FLOAT32 = 10,
-  HC_FLOAT64 = 11
+  HC_FLOAT64 = 11,
+  HC_BFLOAT16 = 12  /* surrogate network activations only */
 } hc_dtype;
 
 /* boundary modes, _vendored/_ndimage_util.py:186-308 ('wrap' == 'grid-wrap',
@@ -228,6 +229,38 @@ int hc_meijering_fold(const hc_array* H, int nH, const hc_array* out, const hc_a
 /* a[a <= threshold] = value (float32 / float64): the last step of the hybrid
  * Hessian filter, filters/ridges.py:479-565. */
 int hc_fill_le(const hc_array* a, double threshold, double value, void* stream);
+
+/* ---- 3-D convolutional surrogate of the Frangi stack (no counterpart in the
+ * reference; it stands in for filters/ridges.py:322-476 and is judged
+ * against the plain kernels).  Valid 3x3x3 convolutions on channels-last
+ * bfloat16 activations, bf16 weights, float32 accumulation on the matrix
+ * cores.  act: 0 none, 1 GELU (erf), 2 clamp to [0, 1]. */
+
+/* bytes of the packed (matrix-core operand order) weights of a layer */
+int hc_cnn3d_packed_size(int ci, int co);
+/* pack float32 weights w[27][ci][co] (tap = (dz*3 + dy)*3 + dx) */
+int hc_cnn3d_pack(const float* w_dev, void* packed_dev, int ci, int co, void* stream);
+/* out (D-2, H-2, W-2, co) = act(conv(in (D, H, W, ci)) + bias); out bfloat16,
+ * or float32 for the single-channel last layer; pre_dev (float32, out's
+ * shape) receives the pre-activation when not NULL (training).  Compiled
+ * layer shapes: 1->24 GELU, 24->24 GELU, 24->1 none / clamp. */
+int hc_cnn3d_conv(const hc_array* in, const hc_array* out, const void* packed,
+                  const float* bias_dev, int act, float* pre_dev, void* stream);
+/* out (D+2p, H+2p, W+2p[, 1]) bfloat16 = boundary-extended float32 in */
+int hc_cnn3d_pad(const hc_array* in, const hc_array* out, int pad, int mode, double cval,
+                 void* stream);
+/* training (float32): g *= gelu'(pre); input / weight / bias gradients of a
+ * layer (w and dw: [27][ci][co]; dw, db accumulate); masked-free MSE
+ * gradient (loss accumulates the mean squared error); Adam step */
+int hc_cnn3d_gelu_bwd(float* g, const float* pre, int64_t n, void* stream);
+int hc_cnn3d_dgrad(const float* dy, const float* w, float* dx, int D, int H, int W, int ci,
+                   int co, void* stream);
+int hc_cnn3d_wgrad(const void* x_bf16, const float* dy, float* dw, float* db, int D, int H,
+                   int W, int ci, int co, void* stream);
+int hc_cnn3d_mse_grad(const float* y, const float* t, float* g, int64_t n, float* loss,
+                      void* stream);
+int hc_cnn3d_adam(float* p, const float* g, float* m, float* v, int64_t n, double lr,
+                  double b1, double b2, double eps, int step, void* stream);
 
 /* rgb2gray: color/colorconv.py:1037-1095 (+ _prepare_colorarray :160-180).
  * in: (..., 3) C-contiguous, any real dtype; out: (...) float32/float64. */
