@@ -39,7 +39,8 @@ def _parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-edge", type=int, default=96,
                    help="edge of the cubic crop timed on the CPU")
-    p.add_argument("--only", choices=["all", "frangi", "correlate", "copy", "batch2d"],
+    p.add_argument("--only", choices=["all", "frangi", "correlate", "copy", "batch2d",
+                                      "surrogate", "batch3d"],
                    default="all",
                    help="restrict the run (profiler passes); 'copy' runs the plain copy "
                         "kernel that calibrates the HBM byte counters")
@@ -260,6 +261,96 @@ def bench_batch2d(steps, cpu_tiles):
     return res
 
 
+SURROGATE_WEIGHTS = os.path.join(ROOT, "cucim_amd", "data", "frangi_surrogate.npz")
+
+
+def bench_surrogate(steps, edge=512, ntubes=320):
+    """Learned Frangi surrogate vs plain Frangi on an edge^3 float32 tube
+    volume (the 256^3 generator at the same tube density: 320 tubes at
+    512^3), sigmas 1..5, black_ridges=False, gamma fixed to the model's.
+    Times the plain fused Frangi, the raw surrogate, and the checked
+    surrogate (sampled per-brick check + plain fallback); rel-L2 and Dice
+    (threshold 0.5 max of plain) against plain."""
+    import numpy as np
+    import torch
+
+    from cucim_amd import synthetic
+    from cucim_amd.skimage.filters.ridges import FrangiPlan
+    from cucim_amd.surrogate import CheckedSurrogate, FrangiSurrogate, dice, rel_l2
+
+    if not os.path.exists(SURROGATE_WEIGHTS):
+        return {"error": "no trained weights"}
+    model = FrangiSurrogate.load(SURROGATE_WEIGHTS)
+    shape = (edge,) * 3
+    vol = synthetic.tube_volume(shape, ntubes=ntubes, seed=4096)
+    nvox = edge ** 3
+    plan = FrangiPlan(shape, np.float32, list(model.sigmas), "reflect")
+    plain = torch.empty_like(vol)
+
+    def run_plain(_):
+        rc = plan.run(vol, plain, gamma=model.gamma, black_ridges=False)
+        if rc != 0:
+            from cucim_amd import _lib
+            _lib.check(rc, "frangi")
+
+    run_plain(0)
+    ms_plain = _events_time(run_plain, steps)
+    sur = model(vol)
+    ms_sur = _events_time(lambda i: model(vol), steps)
+    res = {"workload": f"{edge}^3 float32, {ntubes} tubes, sigmas 1..5, gamma "
+                       f"{model.gamma:.4f} (fixed)",
+           "plain_ms": ms_plain, "plain_voxels_per_s": nvox / (ms_plain * 1e-3),
+           "surrogate_ms": ms_sur, "surrogate_voxels_per_s": nvox / (ms_sur * 1e-3),
+           "surrogate_speedup": ms_plain / ms_sur,
+           "surrogate_rel_l2": rel_l2(sur, plain),
+           "surrogate_dice": dice(sur, plain, 0.5 * float(plain.max())),
+           "tflops_surrogate": 95904 * nvox / (ms_sur * 1e-3) / 1e12}
+    del sur
+    cs = CheckedSurrogate(model, shape, brick=64, tol=0.02)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out, info = cs(vol)
+    torch.cuda.synchronize()
+    ms_chk = (time.perf_counter() - t0) * 1e3
+    res.update({"checked_ms": ms_chk, "checked_voxels_per_s": nvox / (ms_chk * 1e-3),
+                "checked_speedup": ms_plain / ms_chk, "checked_rel_l2": rel_l2(out, plain),
+                "checked_dice": dice(out, plain, 0.5 * float(plain.max())),
+                "check": info})
+    model.release()
+    return res
+
+
+def bench_batch3d(steps, n=200, seed=77):
+    """Batched uneven 3-D volumes: n float32 tube volumes, each edge drawn
+    independently from 48..192, tube density of the 256^3 generator (40
+    tubes per 256^3), noise sigma 0.08; plain Frangi sigmas (1, 2, 3),
+    black_ridges=False, reflect; the whole batch captured in one HIP graph.
+    Per-volume latency: each volume's schedule run alone."""
+    import numpy as np
+    import torch
+
+    from cucim_amd import synthetic
+    from cucim_amd.batch import FrangiBatch3D
+
+    rng = np.random.default_rng(seed)
+    shapes = [tuple(int(v) for v in rng.integers(48, 193, 3)) for _ in range(n)]
+    vols = []
+    for i, s in enumerate(shapes):
+        nt = max(2, int(round(40 * (s[0] * s[1] * s[2]) / 256 ** 3)))
+        vols.append(synthetic.tube_volume(s, ntubes=nt, seed=seed * 1000 + i, noise=0.08))
+    b = FrangiBatch3D(vols, sigmas=(1, 2, 3))
+    g = b.capture()
+    g.replay()
+    ms = _events_time(lambda i: g.replay(), steps)
+    lat = np.asarray(b.latencies_ms())
+    return {"volumes": n, "voxels": b.voxels, "ms_per_batch": ms,
+            "voxels_per_s": b.voxels / (ms * 1e-3),
+            "latency_ms_p50": float(np.percentile(lat, 50)),
+            "latency_ms_p99": float(np.percentile(lat, 99)),
+            "eager_sum_ms": float(lat.sum()),
+            "sigmas": [1, 2, 3]}
+
+
 COPY_BYTES = 256 << 20  # per direction
 
 
@@ -409,6 +500,12 @@ def main():
     if args.only == "copy":
         print(json.dumps({"copy": bench_copy()}))
         return
+    if args.only == "batch3d":
+        print(json.dumps({"batch3d": bench_batch3d(args.steps)}))
+        return
+    if args.only == "surrogate":
+        print(json.dumps({"surrogate": bench_surrogate(max(2, args.steps // 4))}))
+        return
     if args.only == "batch2d":
         print(json.dumps({"batch2d": bench_batch2d(args.steps, 0 if args.no_cpu
                                                    else args.cpu_tiles)}))
@@ -439,9 +536,11 @@ def main():
     corr = None
     if args.only == "all":
         corr = bench_correlate(tuple(int(v) for v in args.corr_sizes.split(",")))
-    batch2d = None
+    batch2d = surrogate = batch3d = None
     if args.only == "all":
         batch2d = bench_batch2d(args.steps, 0 if args.no_cpu else args.cpu_tiles)
+        batch3d = bench_batch3d(max(2, args.steps // 4))
+        surrogate = bench_surrogate(max(2, args.steps // 4))
     cpu, parity = (None, None)
     if not args.no_cpu and args.only == "all":
         cpu, parity = cpu_baseline(vol, args.cpu_edge)
@@ -485,7 +584,8 @@ def main():
         "cpu_baseline": cpu,
         "parity": parity,
         "batch2d": batch2d,
-        "surrogate": None,
+        "batch3d": batch3d,
+        "surrogate": surrogate,
     }
     print(json.dumps(line))
 

@@ -70,6 +70,7 @@ ENTRY_POINTS = (
     "hc_cnn3d_wgrad",
     "hc_cnn3d_mse_grad",
     "hc_cnn3d_adam",
+    "hc_frangi_check",
 )
 
 
@@ -146,6 +147,8 @@ _SIGNATURES = {
     "hc_cnn3d_wgrad": ([_v, _v, _v, _v, _i, _i, _i, _i, _i, _v], _i),
     "hc_cnn3d_mse_grad": ([_v, _v, _v, ctypes.c_int64, _v, _v], _i),
     "hc_cnn3d_adam": ([_v, _v, _v, _v, ctypes.c_int64, _d, _d, _d, _d, _i, _v], _i),
+    "hc_frangi_check": ([_A, _A, _v, _i, _v, _v, _P(ctypes.c_int32), _i, _i, _d, _d, _d, _v,
+                         _v, _v], _i),
 }
 
 _lock = threading.Lock()

@@ -262,6 +262,19 @@ int hc_cnn3d_mse_grad(const float* y, const float* t, float* g, int64_t n, float
 int hc_cnn3d_adam(float* p, const float* g, float* m, float* v, int64_t n, double lr,
                   double b1, double b2, double eps, int step, void* stream);
 
+/* Sampled plain-Frangi check of the surrogate: for every brick, the plain 3-D
+ * Frangi (filters/ridges.py:322-476 with a fixed gamma) evaluated exactly
+ * on up to 64 voxels of one line along axis 0 (lines[4b..4b+3] = x0, y0,
+ * za, zb), placed by the caller where no Gaussian pass reaches the volume's
+ * border; per sigma kc holds the composite kernels SS, SD, DD (full
+ * convolutions of the pass kernels, L taps each, anchor A = 2 * anchor) and
+ * kinfo (offset, L, A).  plain_out: nb x 64 values; rel_out: per brick the
+ * relative L2 error of the surrogate output sur on the samples. */
+int hc_frangi_check(const hc_array* vol, const hc_array* sur, const int32_t* lines_dev, int nb,
+                    const float* kc_dev, const int32_t* kinfo_dev, const int32_t* kinfo_host,
+                    int nsig, int black_ridges, double alpha, double beta, double gamma,
+                    float* plain_out, float* rel_out, void* stream);
+
 /* rgb2gray: color/colorconv.py:1037-1095 (+ _prepare_colorarray :160-180).
  * in: (..., 3) C-contiguous, any real dtype; out: (...) float32/float64. */
 int hc_rgb2gray(const hc_array* in, const hc_array* out, void* stream);

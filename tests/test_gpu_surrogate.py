@@ -167,3 +167,70 @@ def test_surrogate_forward_shape_and_training_descends(gpu):
     xs, ts = sample_patches([vol], [lab], 4, edge=32, seed=0)
     losses = [float(m.train_step(xs, ts, lr=3e-3)) for _ in range(30)]
     assert losses[-1] < 0.7 * losses[0], losses
+
+
+def _sample_positions(cs):
+    lines = cs.lines.cpu().numpy()
+    pos = []
+    for b, (x0, y0, za, zb) in enumerate(lines):
+        for k, z in enumerate(range(za, zb)):
+            pos.append((b, k, z, y0, x0))
+    return pos
+
+
+def test_frangi_check_samples_match_plain(gpu):
+    """The sampled exact plain values equal the plain Frangi (fused kernels,
+    same fixed gamma) at those voxels within float32 rounding."""
+    from cucim_amd import synthetic
+    from cucim_amd.surrogate import CheckedSurrogate, FrangiSurrogate, frangi_label
+
+    vol = synthetic.tube_volume((176, 160, 168), ntubes=12, seed=3)
+    m = FrangiSurrogate(sigmas=(1, 2, 3, 4, 5), gamma=0.17, seed=2)
+    cs = CheckedSurrogate(m, vol.shape, brick=64)
+    sur = m(vol)
+    cs.check(vol, sur)
+    plain = frangi_label(vol, m.sigmas, m.gamma).cpu().numpy()
+    samp = cs.plain_s.cpu().numpy().reshape(-1, 64)
+    pos = _sample_positions(cs)
+    assert len(pos) > 0
+    got = np.array([samp[b, k] for b, k, z, y, x in pos])
+    exp = np.array([plain[z, y, x] for b, k, z, y, x in pos])
+    np.testing.assert_allclose(got, exp, rtol=1e-3, atol=1e-5 * max(1.0, exp.max()))
+    # the per-brick error of the kernel matches the samples
+    rel = cs.rel.cpu().numpy()
+    s = sur.cpu().numpy()
+    for b in range(cs.nb):
+        pb = [(k, z, y, x) for bb, k, z, y, x in pos if bb == b]
+        if not pb:
+            assert np.isnan(rel[b])
+            continue
+        e = np.array([plain[z, y, x] for _, z, y, x in pb])
+        q = np.array([s[z, y, x] for _, z, y, x in pb])
+        den = np.linalg.norm(e)
+        if den > 1e-6:
+            assert abs(rel[b] - np.linalg.norm(q - e) / den) <= 1e-2 * rel[b] + 1e-5
+
+
+@pytest.mark.parametrize("path", ["region", "mass"])
+def test_fallback_is_plain(gpu, path):
+    """Recomputed bricks equal the plain Frangi of the whole volume bit for
+    bit (per-brick regions carry the plain filter's full reach)."""
+    from cucim_amd import synthetic
+    from cucim_amd.surrogate import CheckedSurrogate, FrangiSurrogate, frangi_label
+
+    vol = synthetic.tube_volume((256, 192, 200), ntubes=16, seed=4)
+    m = FrangiSurrogate(sigmas=(1, 2, 3), gamma=0.2, seed=3)
+    cs = CheckedSurrogate(m, vol.shape, brick=64)
+    plain = frangi_label(vol, m.sigmas, m.gamma).cpu().numpy()
+    raw = m(vol).cpu().numpy()
+    force = [0, 5, cs.nb - 1] if path == "region" else list(range(cs.nb))
+    out, info = cs(vol, force=force)
+    assert info["mass_fallback"] == (path == "mass")
+    out = out.cpu().numpy()
+    gz, gy, gx = cs.grid
+    for i in range(cs.nb):
+        bz, by, bx = i // (gy * gx), (i // gx) % gy, i % gx
+        sl = (slice(bz * 64, bz * 64 + 64), slice(by * 64, by * 64 + 64),
+              slice(bx * 64, bx * 64 + 64))
+        ref = plain[sl] if i in force else raw[sl]
+        np.testing.assert_array_equal(out[sl].view(np.uint32), ref.view(np.uint32))
