@@ -338,17 +338,46 @@ def bench_batch3d(steps, n=200, seed=77):
     for i, s in enumerate(shapes):
         nt = max(2, int(round(40 * (s[0] * s[1] * s[2]) / 256 ** 3)))
         vols.append(synthetic.tube_volume(s, ntubes=nt, seed=seed * 1000 + i, noise=0.08))
-    b = FrangiBatch3D(vols, sigmas=(1, 2, 3))
+    res = {"volumes": n, "sigmas": [1, 2, 3]}
+    weights = os.path.join(ROOT, "cucim_amd", "data", "frangi_surrogate_s123.npz")
+    gamma = None
+    sb = None
+    if os.path.exists(weights):
+        from cucim_amd.surrogate import FrangiSurrogate, SurrogateBatch3D, dice, rel_l2
+
+        model = FrangiSurrogate.load(weights)
+        gamma = model.gamma  # plain and surrogate with the same fixed gamma
+        sb = SurrogateBatch3D(model, vols)
+    b = FrangiBatch3D(vols, sigmas=(1, 2, 3), gamma=gamma)
     g = b.capture()
     g.replay()
     ms = _events_time(lambda i: g.replay(), steps)
     lat = np.asarray(b.latencies_ms())
-    return {"volumes": n, "voxels": b.voxels, "ms_per_batch": ms,
-            "voxels_per_s": b.voxels / (ms * 1e-3),
-            "latency_ms_p50": float(np.percentile(lat, 50)),
-            "latency_ms_p99": float(np.percentile(lat, 99)),
-            "eager_sum_ms": float(lat.sum()),
-            "sigmas": [1, 2, 3]}
+    res.update({"voxels": b.voxels, "gamma": gamma if gamma is not None else "per volume",
+                "plain": {"ms_per_batch": ms, "voxels_per_s": b.voxels / (ms * 1e-3),
+                          "latency_ms_p50": float(np.percentile(lat, 50)),
+                          "latency_ms_p99": float(np.percentile(lat, 99)),
+                          "eager_sum_ms": float(lat.sum())}})
+    if sb is not None:
+        gs = sb.capture()
+        gs.replay()
+        ms_s = _events_time(lambda i: gs.replay(), steps)
+        lat_s = []
+        for i in range(len(vols)):
+            lat_s.append(_events_time(lambda j, i=i: sb.run_one(i), 1))
+        lat_s = np.asarray(lat_s)
+        gs.replay()
+        b.run()
+        torch.cuda.synchronize()
+        allp = torch.cat([o.flatten() for o in b.outputs])
+        alls = torch.cat([o.flatten() for o in sb.outputs])
+        res["surrogate"] = {"ms_per_batch": ms_s, "voxels_per_s": b.voxels / (ms_s * 1e-3),
+                            "speedup_vs_plain": ms / ms_s,
+                            "latency_ms_p50": float(np.percentile(lat_s, 50)),
+                            "latency_ms_p99": float(np.percentile(lat_s, 99)),
+                            "rel_l2": rel_l2(alls, allp),
+                            "dice": dice(alls, allp, 0.5 * float(allp.max()))}
+    return res
 
 
 COPY_BYTES = 256 << 20  # per direction

@@ -238,6 +238,89 @@ __device__ __forceinline__ void conv_colP(W (&acc)[NK][P], const W* __restrict__
   if (t < K) blk(va, wa, t);
 }
 
+// 8 consecutive outputs along an LDS row: acc[n][j] += sum_k w_n[k] x[j + k].
+// The window x[t .. t + 11] of each group of 4 taps lives in three of four
+// rotating 4-value registers (one 16-byte LDS read per group), the weights
+// in two rotating sets; reads x[0 .. K + 11] and w[0 .. K + 3].
+template <typename W, int NK, bool SKIP>
+__device__ __forceinline__ void conv_row8n(W (&acc)[NK][8], const W* __restrict__ x,
+                                           const W* const (&wl)[NK],
+                                           const TapPack<W>* const (&tp)[NK], int K) {
+  W ra[4], rb[4], rc[4], rd[4];
+  W wa[NK][4], wb[NK][4];
+  uint64_t mk[NK];
+#pragma unroll
+  for (int n = 0; n < NK; ++n) mk[n] = SKIP ? full_mask(*tp[n]) : 0ull;
+  auto blk = [&](const W (&A)[4], const W (&B)[4], const W (&C)[4], const W (&ww)[NK][4],
+                 int t) {
+    W v[12];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) { v[m] = A[m]; v[m + 4] = B[m]; v[m + 8] = C[m]; }
+#pragma unroll
+    for (int n = 0; n < NK; ++n) group4<W, 8, SKIP>(acc[n], v, ww[n], mk[n], t);
+  };
+  ld4(x, ra);
+  ld4(x + 4, rb);
+  ld4(x + 8, rc);
+#pragma unroll
+  for (int n = 0; n < NK; ++n) ld4(wl[n], wa[n]);
+  int t = 0;
+  for (; t + 16 <= K; t += 16) {
+    ld4(x + t + 12, rd);
+#pragma unroll
+    for (int n = 0; n < NK; ++n) ld4(wl[n] + t + 4, wb[n]);
+    blk(ra, rb, rc, wa, t);
+    ld4(x + t + 16, ra);
+#pragma unroll
+    for (int n = 0; n < NK; ++n) ld4(wl[n] + t + 8, wa[n]);
+    blk(rb, rc, rd, wb, t + 4);
+    ld4(x + t + 20, rb);
+#pragma unroll
+    for (int n = 0; n < NK; ++n) ld4(wl[n] + t + 12, wb[n]);
+    blk(rc, rd, ra, wa, t + 8);
+    ld4(x + t + 24, rc);
+#pragma unroll
+    for (int n = 0; n < NK; ++n) ld4(wl[n] + t + 16, wa[n]);
+    blk(rd, ra, rb, wb, t + 12);
+  }
+  // 0 .. 3 groups left; ra, rb, rc hold x[t .. t + 11], wa holds w[t .. t + 3]
+  for (; t < K; t += 4) {
+    ld4(x + t + 12, rd);
+#pragma unroll
+    for (int n = 0; n < NK; ++n) ld4(wl[n] + t + 4, wb[n]);
+    blk(ra, rb, rc, wa, t);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) { ra[m] = rb[m]; rb[m] = rc[m]; rc[m] = rd[m]; }
+#pragma unroll
+    for (int n = 0; n < NK; ++n)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) wa[n][m] = wb[n][m];
+  }
+}
+
+template <bool SKIP, typename W>
+__device__ __forceinline__ void conv_row8(W (&acc)[8], const W* __restrict__ x,
+                                          const W* __restrict__ wl, const TapPack<W>& tp) {
+  W a[1][8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) a[0][j] = acc[j];
+  const W* const wls[1] = {wl};
+  const TapPack<W>* const tps[1] = {&tp};
+  conv_row8n<W, 1, SKIP>(a, x, wls, tps, tp.K);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc[j] = a[0][j];
+}
+
+// P = 4 or 8 consecutive outputs along an LDS row.
+template <bool SKIP, int P, typename W>
+__device__ __forceinline__ void conv_rowP(W (&acc)[P], const W* __restrict__ x,
+                                          const W* __restrict__ wl, const TapPack<W>& tp) {
+  if constexpr (P == 8)
+    conv_row8<SKIP>(acc, x, wl, tp);
+  else
+    conv_row4<SKIP>(acc, x, wl, tp);
+}
+
 template <bool SKIP, typename W, int S>
 __device__ __forceinline__ void conv_col16(W (&acc)[16], const W* __restrict__ x,
                                            const W* __restrict__ wl, const TapPack<W>& tp) {

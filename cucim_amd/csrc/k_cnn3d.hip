@@ -35,20 +35,26 @@ namespace hc {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kBZ = 2, kBY = 8, kBX = 16;  // output block (z, y, x)
-constexpr int kTZ = kBZ + 2, kTY = kBY + 2, kTX = kBX + 2;  // input tile
+constexpr int kBY = 8, kBX = 16;        // output block rows / columns
+constexpr int kTY = kBY + 2, kTX = kBX + 2;  // input tile rows / columns
 
 __device__ __forceinline__ float bf2f(__bf16 v) { return (float)v; }
 __device__ __forceinline__ __bf16 f2bf(float v) { return (__bf16)v; }  // RNE
 
-// exact (erf) GELU, as torch.nn.GELU()
+// GELU in its tanh form (torch.nn.GELU(approximate="tanh")):
+// 0.5 x (1 + tanh(z / 2)) = x * sigmoid(z), z = 2 sqrt(2/pi) (x + 0.044715 x^3),
+// with the hardware exp and reciprocal (the network is trained with this
+// same function and its derivative).
+__device__ __forceinline__ float gelu_z(float x) {
+  return 1.5957691216057308f * fmaf(0.044715f * x, x * x, x);
+}
 __device__ __forceinline__ float gelu(float x) {
-  return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
+  return x * __builtin_amdgcn_rcpf(1.0f + __expf(-gelu_z(x)));
 }
 __device__ __forceinline__ float gelu_grad(float x) {
-  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-  const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
-  return cdf + x * pdf;
+  const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-gelu_z(x)));
+  const float dz = 1.5957691216057308f * fmaf(3.0f * 0.044715f, x * x, 1.0f);
+  return fmaf(x * s * (1.0f - s), dz, s);
 }
 
 enum { ACT_NONE = 0, ACT_GELU = 1, ACT_CLAMP01 = 2 };
@@ -70,37 +76,66 @@ struct ConvGeom {
   static constexpr int NCH = (K + 31) / 32;  // K chunks of 32
 };
 
-template <int CI, int CO, int ACT, bool OUT_F32>
-__global__ __launch_bounds__(256) void k_conv3d_mfma(const ConvArgs a) {
+// Block b of the output grid (x fastest) -> origin.
+template <int BZ>
+__device__ __forceinline__ void block_origin(const ConvArgs& a, int64_t b, int& x0, int& y0,
+                                             int& z0) {
+  x0 = (int)(b % a.nbx) * kBX;
+  y0 = (int)((b / a.nbx) % a.nby) * kBY;
+  z0 = (int)(b / ((int64_t)a.nbx * a.nby)) * BZ;
+}
+
+// BZ output planes per block, two waves per plane (4 rows of 16 voxels each).
+template <int CI, int CO, int ACT, bool OUT_F32, int BZ>
+__global__ __launch_bounds__(128 * BZ) void k_conv3d_mfma(const ConvArgs a) {
+  constexpr int NTH = 128 * BZ;
+  constexpr int kBZ = BZ, kTZ = BZ + 2;
   constexpr int NCH = ConvGeom<CI>::NCH;
   constexpr int NT = (CO + 15) / 16;
-  constexpr int TILE = kTZ * kTY * kTX * CI;  // bf16 elements
+  constexpr int TVOX = kTZ * kTY * kTX;
+  constexpr int TILE = TVOX * CI;                   // bf16 elements
+  constexpr bool VEC = CI % 8 == 0;
+  constexpr int TV = VEC ? TVOX * (CI / 8) : 1;     // 16-byte vectors of a tile
+  constexpr int NPF = VEC ? (TV + NTH - 1) / NTH : 1;  // prefetch registers per lane
+  constexpr bool STAGE_OUT = !OUT_F32 && CO % 8 == 0;
   extern __shared__ __align__(16) unsigned char smem[];
-  bf16x8* Wl = reinterpret_cast<bf16x8*>(smem);               // NCH * NT * 64
+  bf16x8* Wl = reinterpret_cast<bf16x8*>(smem);   // NCH * NT * 64
   __bf16* T = reinterpret_cast<__bf16*>(smem + NCH * NT * 64 * 16);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  // weights -> LDS
-  for (int i = tid; i < NCH * NT * 64; i += 256) Wl[i] = a.wpk[i];
-  const int64_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int bx = (int)(blk % a.nbx);
-  const int by = (int)((blk / a.nbx) % a.nby);
-  const int bz = (int)(blk / ((int64_t)a.nbx * a.nby));
-  const int x0 = bx * kBX, y0 = by * kBY, z0 = bz * kBZ;
-  // input tile -> LDS (rows of kTX voxels x CI channels are contiguous)
-  if (CI % 8 == 0) {
-    constexpr int V = CI / 8;  // 16-byte vectors per voxel
-    for (int e = tid; e < kTZ * kTY * kTX * V; e += 256) {
-      const int v = e % V, vox = e / V;
-      const int tx = vox % kTX, ty = (vox / kTX) % kTY, tz = vox / (kTX * kTY);
-      const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
+  for (int i = tid; i < NCH * NT * 64; i += NTH) Wl[i] = a.wpk[i];
+  const int64_t G = gridDim.x;
+  const int64_t slot = xcd_swizzle(blockIdx.x, G);
+  // input tile of block b: global -> registers (VEC) or straight to LDS
+  auto fetch = [&](int64_t b, bf16x8 (&r)[NPF]) {
+    int x0, y0, z0;
+    block_origin<BZ>(a, b, x0, y0, z0);
+#pragma unroll
+    for (int u = 0; u < NPF; ++u) {
+      const int e = tid + NTH * u;
       bf16x8 val = {};
-      if (gx < a.W && gy < a.H && gz < a.D)
-        val = reinterpret_cast<const bf16x8*>(
-            a.in + (((int64_t)gz * a.H + gy) * a.W + gx) * CI)[v];
-      reinterpret_cast<bf16x8*>(T)[e] = val;
+      if (e < TV) {
+        constexpr int V = VEC ? CI / 8 : 1;
+        const int v = e % V, vox = e / V;
+        const int tx = vox % kTX, ty = (vox / kTX) % kTY, tz = vox / (kTX * kTY);
+        const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
+        if (gx < a.W && gy < a.H && gz < a.D)
+          val = reinterpret_cast<const bf16x8*>(
+              a.in + (((int64_t)gz * a.H + gy) * a.W + gx) * CI)[v];
+      }
+      r[u] = val;
     }
-  } else {
-    for (int e = tid; e < TILE; e += 256) {
+  };
+  auto put = [&](const bf16x8 (&r)[NPF]) {
+#pragma unroll
+    for (int u = 0; u < NPF; ++u) {
+      const int e = tid + NTH * u;
+      if (e < TV) reinterpret_cast<bf16x8*>(T)[e] = r[u];
+    }
+  };
+  auto stage_scalar = [&](int64_t b) {
+    int x0, y0, z0;
+    block_origin<BZ>(a, b, x0, y0, z0);
+    for (int e = tid; e < TILE; e += NTH) {
       const int c = e % CI, vox = e / CI;
       const int tx = vox % kTX, ty = (vox / kTX) % kTY, tz = vox / (kTX * kTY);
       const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
@@ -109,81 +144,125 @@ __global__ __launch_bounds__(256) void k_conv3d_mfma(const ConvArgs a) {
         val = a.in[(((int64_t)gz * a.H + gy) * a.W + gx) * CI + c];
       T[e] = val;
     }
-  }
-  __syncthreads();
-  // wave wv: z = wv / 2, rows y = 4 (wv % 2) .. +3; M-tile m = one row of 16 x
+  };
+  bf16x8 pf[NPF];
+  if (VEC && slot < a.nblocks) fetch(slot, pf);
+  const int OD = a.D - 2, OH = a.H - 2, OW = a.W - 2;
   const int oz = wv >> 1, oy0 = (wv & 1) * 4;
-  f32x4 acc[4][NT];
-#pragma unroll
-  for (int m = 0; m < 4; ++m)
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int xi = lane & 15, kg = lane >> 4;  // A row (voxel x), K group
-#pragma unroll 1
-  for (int c = 0; c < NCH; ++c) {
-    bf16x8 b[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) b[t] = Wl[(c * NT + t) * 64 + lane];
-    bf16x8 av[4];
-    if (CI % 8 == 0) {
-      const int k = 32 * c + 8 * kg;  // first K index of this lane's group
-      const int tap = k / CI, ci0 = k - tap * CI;
-      const bool valid = tap < 27;
-      const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-#pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        const int vox = ((oz + dz) * kTY + (oy0 + m + dy)) * kTX + (xi + dx);
-        bf16x8 v = {};
-        if (valid) v = *reinterpret_cast<const bf16x8*>(T + vox * CI + ci0);
-        av[m] = v;
-      }
+  for (int64_t base = 0; base < a.nblocks; base += G) {
+    const int64_t blk = base + slot;
+    if (blk >= a.nblocks) break;  // uniform per work-group
+    int x0, y0, z0;
+    block_origin<BZ>(a, blk, x0, y0, z0);
+    __syncthreads();  // the previous block is done with T
+    if (VEC) {
+      put(pf);
+      if (blk + G < a.nblocks) fetch(blk + G, pf);  // in flight during the MFMA loop
     } else {
-      // CI == 1: a lane's 8 K values are 8 consecutive taps of one voxel
-#pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        bf16x8 v = {};
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int tap = 32 * c + 8 * kg + j;
-          if (tap < 27) {
-            const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-            v[j] = T[((oz + dz) * kTY + (oy0 + m + dy)) * kTX + (xi + dx)];
-          }
-        }
-        av[m] = v;
-      }
+      stage_scalar(blk);
     }
+    __syncthreads();
+    f32x4 acc[4][NT];
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
-      for (int t = 0; t < NT; ++t)
-        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], b[t], acc[m][t], 0, 0, 0);
-  }
-  // epilogue: D[4 (l/16) + r][l % 16] = (x = 4 kg + r, channel 16 t + l % 16)
-  const int OD = a.D - 2, OH = a.H - 2, OW = a.W - 2;
-  const int gz = z0 + oz;
+      for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // fragments of K chunk c: B from the packed weights, A from the tile
+    auto frag = [&](int c, bf16x8 (&av)[4], bf16x8 (&bw)[NT]) {
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    const int gy = y0 + oy0 + m;
+      for (int t = 0; t < NT; ++t) bw[t] = Wl[(c * NT + t) * 64 + lane];
+      if (VEC) {
+        const int k = 32 * c + 8 * kg;  // first K index of this lane's group
+        const int tap = k / CI, ci0 = k - tap * CI;
+        const bool valid = tap < 27;
+        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int co = 16 * t + (lane & 15);
-      if (co >= CO) continue;
-      const float bias = a.bias[co];
+        for (int m = 0; m < 4; ++m) {
+          const int vox = ((oz + dz) * kTY + (oy0 + m + dy)) * kTX + (xi + dx);
+          bf16x8 v = {};
+          if (valid) v = *reinterpret_cast<const bf16x8*>(T + vox * CI + ci0);
+          av[m] = v;
+        }
+      } else {
+        // CI == 1: a lane's 8 K values are 8 consecutive taps of one voxel
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gx = x0 + 4 * kg + r;
-        if (gx >= OW || gy >= OH || gz >= OD) continue;
-        const int64_t o = (((int64_t)gz * OH + gy) * OW + gx) * CO + co;
-        const float p = acc[m][t][r] + bias;
-        if (a.pre) a.pre[o] = p;
-        float y = p;
-        if (ACT == ACT_GELU) y = gelu(p);
-        if (ACT == ACT_CLAMP01) y = fminf(fmaxf(p, 0.f), 1.f);
-        if (OUT_F32)
-          reinterpret_cast<float*>(a.out)[o] = y;
-        else
-          reinterpret_cast<__bf16*>(a.out)[o] = f2bf(y);
+        for (int m = 0; m < 4; ++m) {
+          bf16x8 v = {};
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int tap = 32 * c + 8 * kg + j;
+            if (tap < 27) {
+              const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+              v[j] = T[((oz + dz) * kTY + (oy0 + m + dy)) * kTX + (xi + dx)];
+            }
+          }
+          av[m] = v;
+        }
+      }
+    };
+    auto mma = [&](const bf16x8 (&av)[4], const bf16x8 (&bw)[NT]) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bw[t], acc[m][t], 0, 0, 0);
+    };
+    // two fragment sets: the loads of chunk c + 1 are in flight while the
+    // matrix instructions of chunk c run
+    bf16x8 a0[4], b0[NT], a1[4], b1[NT];
+    frag(0, a0, b0);
+#pragma unroll 1
+    for (int c = 0; c < NCH; c += 2) {
+      if (c + 1 < NCH) frag(c + 1, a1, b1);
+      mma(a0, b0);
+      if (c + 2 < NCH) frag(c + 2, a0, b0);
+      if (c + 1 < NCH) mma(a1, b1);
+    }
+    // epilogue: D[4 (l/16) + r][l % 16] = (x = 4 kg + r, channel 16 t + l % 16)
+    const int gz = z0 + oz;
+    __bf16* O = T;  // output block staging (kBZ * kBY * kBX voxels x CO)
+    if (STAGE_OUT) __syncthreads();  // every wave is done reading T
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int gy = y0 + oy0 + m;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int co = 16 * t + (lane & 15);
+        if (co >= CO) continue;
+        const float bias = a.bias[co];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int lx = 4 * kg + r, gx = x0 + lx;
+          const bool inside = gx < OW && gy < OH && gz < OD;
+          const float p = acc[m][t][r] + bias;
+          float y = p;
+          if (ACT == ACT_GELU) y = gelu(p);
+          if (ACT == ACT_CLAMP01) y = fminf(fmaxf(p, 0.f), 1.f);
+          const int64_t o = (((int64_t)gz * OH + gy) * OW + gx) * CO + co;
+          if (inside && a.pre) a.pre[o] = p;
+          if (STAGE_OUT) {
+            O[((oz * kBY + oy0 + m) * kBX + lx) * CO + co] = f2bf(y);
+          } else if (inside) {
+            if (OUT_F32)
+              reinterpret_cast<float*>(a.out)[o] = y;
+            else
+              reinterpret_cast<__bf16*>(a.out)[o] = f2bf(y);
+          }
+        }
+      }
+    }
+    if (STAGE_OUT) {
+      __syncthreads();
+      constexpr int VPV = CO / 8;  // 16-byte vectors per voxel
+      for (int e = tid; e < kBZ * kBY * kBX * VPV; e += NTH) {
+        const int v = e % VPV, vox = e / VPV;
+        const int lx = vox % kBX, ly = (vox / kBX) % kBY, lz = vox / (kBX * kBY);
+        const int gx = x0 + lx, gy = y0 + ly, gzz = z0 + lz;
+        if (gx < OW && gy < OH && gzz < OD)
+          reinterpret_cast<bf16x8*>(reinterpret_cast<__bf16*>(a.out) +
+                                    (((int64_t)gzz * OH + gy) * OW + gx) * CO)[v] =
+              reinterpret_cast<const bf16x8*>(O)[e];
       }
     }
   }
@@ -347,18 +426,30 @@ namespace {
 
 int64_t grid_of(int64_t n) { return n <= 0 ? 1 : std::min<int64_t>((n + 255) / 256, 256 * 64); }
 
-template <int CI, int CO, int ACT, bool F32>
-int conv_launch(const ConvArgs& a, hipStream_t st) {
+template <int CI, int CO, int ACT, bool F32, int BZ>
+int conv_launch(ConvArgs a, hipStream_t st) {
   constexpr int NCH = ConvGeom<CI>::NCH, NT = (CO + 15) / 16;
-  const size_t smem = (size_t)NCH * NT * 64 * 16 + (size_t)kTZ * kTY * kTX * CI * 2;
+  a.nbx = (a.W - 2 + kBX - 1) / kBX;
+  a.nby = (a.H - 2 + kBY - 1) / kBY;
+  const int nbz = (a.D - 2 + BZ - 1) / BZ;
+  a.nblocks = (int64_t)a.nbx * a.nby * nbz;
+  if (a.nblocks > 0x7fffffff) return fail(HC_ERR_VALUE, "volume too large");
+  // weights, then the input tile (also the output block's staging area)
+  const size_t tile =
+      std::max<size_t>((size_t)(BZ + 2) * kTY * kTX * CI, (size_t)BZ * kBY * kBX * CO);
+  const size_t smem = (size_t)NCH * NT * 64 * 16 + tile * 2;
   if (smem > 64 * 1024) {
-    const hipError_t e = hipFuncSetAttribute((const void*)k_conv3d_mfma<CI, CO, ACT, F32>,
+    const hipError_t e = hipFuncSetAttribute((const void*)k_conv3d_mfma<CI, CO, ACT, F32, BZ>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
                                              (int)smem);
     if (e != hipSuccess) return hip_check(e, "hipFuncSetAttribute(k_conv3d_mfma)");
   }
-  HC_LAUNCH("k_conv3d_mfma", st, (k_conv3d_mfma<CI, CO, ACT, F32>), dim3((unsigned)a.nblocks),
-            dim3(256), smem, a);
+  // persistent: as many work-groups as fit the CUs' LDS, each walking blocks
+  // with the next block's input prefetched during the matrix loop
+  const int per_cu = std::max<int>(1, (int)((160 * 1024) / smem));
+  const int64_t grid = std::min<int64_t>(a.nblocks, (int64_t)256 * per_cu);
+  HC_LAUNCH("k_conv3d_mfma", st, (k_conv3d_mfma<CI, CO, ACT, F32, BZ>), dim3((unsigned)grid),
+            dim3(128 * BZ), smem, a);
   return HC_OK;
 }
 
@@ -407,22 +498,17 @@ int hc_cnn3d_conv(const hc_array* in, const hc_array* out, const void* packed,
   a.D = (int)in->shape[0];
   a.H = (int)in->shape[1];
   a.W = (int)in->shape[2];
-  a.nbx = (a.W - 2 + kBX - 1) / kBX;
-  a.nby = (a.H - 2 + kBY - 1) / kBY;
-  const int nbz = (a.D - 2 + kBZ - 1) / kBZ;
-  a.nblocks = (int64_t)a.nbx * a.nby * nbz;
-  if (a.nblocks > 0x7fffffff) return fail(HC_ERR_VALUE, "volume too large");
   hipStream_t st = (hipStream_t)stream;
   const bool f32 = out->dtype == HC_FLOAT32;
   if (!f32 && out->dtype != HC_BFLOAT16) return fail(HC_ERR_VALUE, "output must be bf16 or f32");
   if (ci == 1 && co == 24 && act == ACT_GELU && !f32)
-    return conv_launch<1, 24, ACT_GELU, false>(a, st);
+    return conv_launch<1, 24, ACT_GELU, false, 2>(a, st);
   if (ci == 24 && co == 24 && act == ACT_GELU && !f32)
-    return conv_launch<24, 24, ACT_GELU, false>(a, st);
+    return conv_launch<24, 24, ACT_GELU, false, 4>(a, st);
   if (ci == 24 && co == 1 && act == ACT_CLAMP01 && f32)
-    return conv_launch<24, 1, ACT_CLAMP01, true>(a, st);
+    return conv_launch<24, 1, ACT_CLAMP01, true, 4>(a, st);
   if (ci == 24 && co == 1 && act == ACT_NONE && f32)
-    return conv_launch<24, 1, ACT_NONE, true>(a, st);
+    return conv_launch<24, 1, ACT_NONE, true, 4>(a, st);
   return fail(HC_ERR_UNSUPPORTED, "layer shape / activation not compiled");
 }
 

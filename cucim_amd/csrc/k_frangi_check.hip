@@ -1,9 +1,10 @@
 // Sampled plain-Frangi check of the surrogate (gfx950).
 //
-// For every brick of a volume, the plain 3-D Frangi vesselness
-// (python/cucim/src/cucim/skimage/filters/ridges.py:322-476, black_ridges
-// as given, gamma fixed) is evaluated exactly on up to 64 voxels of one line
-// along axis 0, and compared with the surrogate's output there.
+// For every sampled line of a volume (one work-group each), the plain 3-D
+// Frangi vesselness (python/cucim/src/cucim/skimage/filters/ridges.py:322-476,
+// black_ridges as given, gamma fixed) is evaluated exactly on up to 64 voxels
+// along axis 0 and compared with the surrogate's output there: the line's
+// sum of squared differences, sum of squared plain values and sample count.
 //
 // Per sigma the Hessian element H_ab = G_b(G_a(I)) is, along each axis, a
 // correlation with the composite 1-D kernel k1 * k2 (full convolution of the
@@ -37,8 +38,8 @@ struct CheckArgs {
   int nsig;
   float sign;            // -1: black_ridges=False (the image is negated)
   double alpha_sq, beta_sq, gamma_sq;
-  float* plain_out;      // per brick: kChkSamples values (NaN where unused)
-  float* rel_out;        // per brick: sampled relative L2 error (NaN: no samples)
+  float* plain_out;      // per line: kChkSamples values (NaN where unused)
+  float* err_out;        // per line: sum (sur - plain)^2, sum plain^2, samples
 };
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -136,9 +137,9 @@ __global__ __launch_bounds__(256) void k_frangi_check(const CheckArgs a) {
     dn = wave_sum(dn);
     nn = wave_sum(nn);
     if (lane == 0) {
-      float r = __builtin_nanf("");
-      if (ns > 0) r = nn > 0.f ? sqrtf(dn / nn) : (dn > 0.f ? __builtin_inff() : 0.f);
-      a.rel_out[b] = r;
+      a.err_out[3 * (int64_t)b] = dn;
+      a.err_out[3 * (int64_t)b + 1] = nn;
+      a.err_out[3 * (int64_t)b + 2] = (float)(ns > 0 ? ns : 0);
     }
   }
 }
@@ -152,9 +153,9 @@ extern "C" {
 int hc_frangi_check(const hc_array* vol, const hc_array* sur, const int32_t* lines_dev, int nb,
                     const float* kc_dev, const int32_t* kinfo_dev, const int32_t* kinfo_host,
                     int nsig, int black_ridges, double alpha, double beta, double gamma,
-                    float* plain_out, float* rel_out, void* stream) {
+                    float* plain_out, float* err_out, void* stream) {
   if (!vol || !sur || !lines_dev || !kc_dev || !kinfo_dev || !kinfo_host || !plain_out ||
-      !rel_out)
+      !err_out)
     return fail(HC_ERR_VALUE, "null argument");
   if (vol->dtype != HC_FLOAT32 || vol->ndim != 3 || !is_c_contiguous(vol) ||
       !same_shape(vol, sur) || sur->dtype != HC_FLOAT32 || !is_c_contiguous(sur))
@@ -181,7 +182,7 @@ int hc_frangi_check(const hc_array* vol, const hc_array* sur, const int32_t* lin
   a.beta_sq = 2 * beta * beta;
   a.gamma_sq = 2 * gamma * gamma;
   a.plain_out = plain_out;
-  a.rel_out = rel_out;
+  a.err_out = err_out;
   hipStream_t st = (hipStream_t)stream;
   HC_LAUNCH("k_frangi_check", st, k_frangi_check, dim3((unsigned)nb), dim3(256), 0, a);
   return HC_OK;

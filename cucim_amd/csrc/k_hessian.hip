@@ -31,7 +31,8 @@ namespace hc {
 
 constexpr int kChC = 16;        // strided chain pass: lines per work-group
 constexpr int kChS = kChC + 1;  // LDS row stride
-constexpr int kFinIPT = 2;      // final pass: 4-output items per thread
+constexpr int kFinIPT = 1;      // final pass: items per thread
+constexpr int kFinP = 8;        // final pass: consecutive outputs per item
 constexpr int kMaxLds = 160 * 1024;
 
 struct ChainOut {
@@ -374,11 +375,11 @@ __device__ __forceinline__ void atomic_max_bits<double>(void* p, double v) {
 
 // One input of the final pass: stage its rows, first op, re-extend, second
 // op into res (IPT items of 4 outputs per thread).
-template <typename W, int IPT, int J>
+template <typename W, int IPT, int P, int J>
 __device__ __forceinline__ void final_input(const FinalArgs<W>& a, W* xin, W* xf, const W* wS,
-                                            const W* wD, int64_t row0, W (&res)[IPT][4]) {
+                                            const W* wD, int64_t row0, W (&res)[IPT][P]) {
   const int E = a.E, R = a.rows, anc = a.S.anchor, tid = threadIdx.x;
-  const int nq = a.Nr / 4, nitems = R * nq;
+  const int nq = a.Nr / P, nitems = R * nq;
   bool bad;
   {
     const int64_t N = a.N, nrows = a.nrows;
@@ -403,11 +404,13 @@ __device__ __forceinline__ void final_input(const FinalArgs<W>& a, W* xin, W* xf
     for (int i = 0; i < IPT; ++i) {
       const int it = tid + 256 * i;
       if (it < nitems) {
-        const int rr = it / nq, o = (it - rr * nq) * 4;
-        W acc[4] = {(W)0, (W)0, (W)0, (W)0};
-        conv_row4<SK>(acc, xin + rr * E + o, w1, t1);
+        const int rr = it / nq, o = (it - rr * nq) * P;
+        W acc[P];
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
+        for (int m = 0; m < P; ++m) acc[m] = (W)0;
+        conv_rowP<SK, P>(acc, xin + rr * E + o, w1, t1);
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
           xf[rr * E + anc + o + m] = acc[m];
           bad1 |= !finite(acc[m]);
         }
@@ -431,24 +434,24 @@ __device__ __forceinline__ void final_input(const FinalArgs<W>& a, W* xin, W* xf
     for (int i = 0; i < IPT; ++i) {
       const int it = tid + 256 * i;
 #pragma unroll
-      for (int m = 0; m < 4; ++m) res[i][m] = (W)0;
+      for (int m = 0; m < P; ++m) res[i][m] = (W)0;
       if (it < nitems) {
-        const int rr = it / nq, o = (it - rr * nq) * 4;
-        conv_row4<SK>(res[i], xf + rr * E + o, w2, t2);
+        const int rr = it / nq, o = (it - rr * nq) * P;
+        conv_rowP<SK, P>(res[i], xf + rr * E + o, w2, t2);
       }
     }
   });
   __syncthreads();
 }
 
-template <typename W, int NIN, int IPT, size_t... Js>
+template <typename W, int NIN, int IPT, int P, size_t... Js>
 __device__ __forceinline__ void final_all(const FinalArgs<W>& a, W* xin, W* xf, const W* wS,
-                                          const W* wD, int64_t row0, W (&res)[NIN][IPT][4],
+                                          const W* wD, int64_t row0, W (&res)[NIN][IPT][P],
                                           std::index_sequence<Js...>) {
-  (final_input<W, IPT, (int)Js>(a, xin, xf, wS, wD, row0, res[Js]), ...);
+  (final_input<W, IPT, P, (int)Js>(a, xin, xf, wS, wD, row0, res[Js]), ...);
 }
 
-template <typename W, int NIN, int IPT>
+template <typename W, int NIN, int IPT, int P>
 __global__ __launch_bounds__(256) void k_final_contig(const FinalArgs<W> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int E = a.E, R = a.rows;
@@ -458,13 +461,13 @@ __global__ __launch_bounds__(256) void k_final_contig(const FinalArgs<W> a) {
   W* wD = wS + a.S.K + 4;
   const int64_t row0 = xcd_swizzle(blockIdx.x, gridDim.x) * R;
   const int tid = threadIdx.x;
-  const int nq = a.Nr / 4;
+  const int nq = a.Nr / P;
   const int nitems = R * nq;
   stage_taps<W>(wS, a.S, tid, 256);
   stage_taps<W>(wD, a.D, tid, 256);
-  W res[NIN][IPT][4];
-  final_all<W, NIN, IPT>(a, xin, xf, wS, wD, row0, res, std::make_index_sequence<NIN>{});
-  // epilogue: every item holds all Hessian elements of its 4 voxels
+  W res[NIN][IPT][P];
+  final_all<W, NIN, IPT, P>(a, xin, xf, wS, wD, row0, res, std::make_index_sequence<NIN>{});
+  // epilogue: every item holds all Hessian elements of its P voxels
   double gsq = a.gamma_sq;
   if (a.epi == EPI_FRANGI && a.gamma_sq_dev) gsq = *a.gamma_sq_dev;
   W rgmax = (W)0;
@@ -472,11 +475,11 @@ __global__ __launch_bounds__(256) void k_final_contig(const FinalArgs<W> a) {
   for (int i = 0; i < IPT; ++i) {
     const int it = tid + 256 * i;
     if (it >= nitems) continue;
-    const int rr = it / nq, o = (it - rr * nq) * 4;
+    const int rr = it / nq, o = (it - rr * nq) * P;
     const int64_t row = row0 + rr;
     if (row >= a.nrows) continue;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
+    for (int m = 0; m < P; ++m) {
       if (o + m >= a.N) continue;
       const int64_t idx = row * a.N + o + m;
       if (a.epi == EPI_STORE) {
@@ -714,11 +717,11 @@ int prepare_final(FinalArgs<W>* a, const hc_array* shape_src, const AxisTaps<W>&
   const Lines ln = lines_of(shape_src, axis);
   a->N = ln.N;
   a->nrows = ln.outer;
-  a->Nr = (int)((ln.N + 3) / 4 * 4);
-  a->E = a->Nr + tp.S.K + 4;
-  if (a->Nr / 4 > kFinIPT * 256)
+  a->Nr = (int)((ln.N + kFinP - 1) / kFinP * kFinP);
+  a->E = a->Nr + tp.S.K + 4;  // the row core reads K + 11 past an item's start
+  if (a->Nr / kFinP > kFinIPT * 256)
     return fail(HC_ERR_RESOURCE, "last axis too long for the fused Hessian epilogue");
-  a->rows = pick_rows<W>(a->E, a->Nr, 2, kFinIPT * 256);
+  a->rows = pick_rows<W>(a->E, a->Nr, 2, kFinIPT * 256 * kFinP / 4);
   if (2 * a->rows * a->E * (int)sizeof(W) > kMaxLds)
     return fail(HC_ERR_RESOURCE, "last axis too long for the fused Hessian epilogue");
   a->mode = mode;
@@ -733,11 +736,11 @@ int launch_final(const FinalArgs<W>& a, hipStream_t st) {
   const int lds = (2 * a.rows * a.E + 2 * (a.S.K + 4)) * (int)sizeof(W);
   const int64_t blocks = (a.nrows + a.rows - 1) / a.rows;
   if (a.nin == 6)
-    HC_LAUNCH("k_final_contig", st, (k_final_contig<W, 6, kFinIPT>), dim3((unsigned)blocks),
-              dim3(256), lds, a);
+    HC_LAUNCH("k_final_contig", st, (k_final_contig<W, 6, kFinIPT, kFinP>),
+              dim3((unsigned)blocks), dim3(256), lds, a);
   else
-    HC_LAUNCH("k_final_contig", st, (k_final_contig<W, 3, kFinIPT>), dim3((unsigned)blocks),
-              dim3(256), lds, a);
+    HC_LAUNCH("k_final_contig", st, (k_final_contig<W, 3, kFinIPT, kFinP>),
+              dim3((unsigned)blocks), dim3(256), lds, a);
   HC_HIP(hipGetLastError());
   return HC_OK;
 }
@@ -754,8 +757,8 @@ int plan_fits(const hc_array* img, const AxisTaps<W>* tp) {
       return fail(HC_ERR_RESOURCE, "line too long for the fused Hessian");
   }
   const int64_t N = img->shape[nd - 1];
-  const int Nr = (int)((N + 3) / 4 * 4);
-  if (Nr / 4 > kFinIPT * 256)
+  const int Nr = (int)((N + kFinP - 1) / kFinP * kFinP);
+  if (Nr / kFinP > kFinIPT * 256)
     return fail(HC_ERR_RESOURCE, "last axis too long for the fused Hessian epilogue");
   const int E = Nr + tp[nd - 1].S.K + 4;
   if (2 * E * (int)sizeof(W) > kMaxLds)

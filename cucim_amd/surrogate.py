@@ -1,7 +1,8 @@
 """Learned surrogate of the multi-scale Frangi stack (3-D CNN on the matrix
 cores).
 
-Five valid 3x3x3 convolutions, 24 channels, GELU between layers, bf16
+Five valid 3x3x3 convolutions, 24 channels, GELU (tanh form) between
+layers, bf16
 weights and activations with float32 accumulation
 (``hc_cnn3d_conv``); the receptive field is 11^3, so a volume is padded by
 5 voxels in the Frangi call's boundary mode and the network's output has the
@@ -103,7 +104,7 @@ class FrangiSurrogate:
                                        _mode_code(mode), float(cval), _array.stream()), "pad")
         return out
 
-    def forward(self, x, keep=False, final_act=2, bufs=None):
+    def forward(self, x, keep=False, final_act=2, bufs=None, out=None):
         """x: (D, H, W, 1) bf16 -> (D-10, H-10, W-10) float32.  keep: return
         the per-layer inputs and pre-activations (training)."""
         lib = _lib.lib()
@@ -113,21 +114,22 @@ class FrangiSurrogate:
             D, H, W = (int(s) - 2 for s in cur.shape[:3])
             last = i == len(LAYERS) - 1
             if last:
-                out = torch.empty((D, H, W, 1), dtype=torch.float32, device=_dev())
+                dst = out.view(D, H, W, 1) if out is not None else \
+                    torch.empty((D, H, W, 1), dtype=torch.float32, device=_dev())
                 act = final_act
             elif bufs is not None:
-                out = bufs[i % 2][: D * H * W * co].view(D, H, W, co)
+                dst = bufs[i % 2][: D * H * W * co].view(D, H, W, co)
             else:
-                out = torch.empty((D, H, W, co), dtype=torch.bfloat16, device=_dev())
+                dst = torch.empty((D, H, W, co), dtype=torch.bfloat16, device=_dev())
             pre = torch.empty((D, H, W, co), dtype=torch.float32, device=_dev()) if keep else None
-            _check(lib.hc_cnn3d_conv(_array.hc_array(cur), _array.hc_array(out),
+            _check(lib.hc_cnn3d_conv(_array.hc_array(cur), _array.hc_array(dst),
                                      self.packed[i].data_ptr(), self.b[i].data_ptr(), act,
                                      None if pre is None else pre.data_ptr(),
                                      _array.stream()), "conv3d")
             if keep:
                 pres.append(pre)
-                acts.append(out)
-            cur = out
+                acts.append(dst)
+            cur = dst
         y = cur.view(cur.shape[:3])
         return (y, acts, pres) if keep else y
 
@@ -333,10 +335,13 @@ def _main(argv=None):
     p.add_argument("--batch", type=int, default=8)
     p.add_argument("--lr", type=float, default=2e-3)
     p.add_argument("--max-seconds", type=float, default=None)
+    p.add_argument("--sigmas", default="1,2,3,4,5")
     a = p.parse_args(argv)
     torch.cuda.set_device(0)
+    sig = tuple(float(v) for v in a.sigmas.split(","))
     model, hist = train(n_patches=a.patches, batch=a.batch, epochs=a.epochs, lr=a.lr,
-                        log=lambda s: print(s, flush=True), max_seconds=a.max_seconds)
+                        sigmas=sig, log=lambda s: print(s, flush=True),
+                        max_seconds=a.max_seconds)
     model.save(a.out)
     from . import synthetic
 
@@ -368,23 +373,26 @@ def _composite_kernels(sigmas):
 
 class CheckedSurrogate:
     """Surrogate Frangi of a (D, H, W) float32 volume with the per-brick plain
-    fallback: every brick (``brick``^3) is checked on up to 64 voxels of one
-    line against the plain Frangi evaluated exactly there
-    (``hc_frangi_check``); a brick whose sampled relative L2 error exceeds
-    ``tol`` -- or that has no sample far enough from the volume's border -- is
-    recomputed with the plain kernels on the brick plus the plain filter's
-    full reach (exact), or, when that would cost more than one plain pass
-    over the volume, copied from one such pass."""
+    fallback.  Every brick (``brick``^3) is checked on up to four lines of 64
+    voxels along axis 0 against the plain Frangi evaluated exactly there
+    (``hc_frangi_check``): its sampled relative L2 error is
+    ||sur - plain|| / max(||plain||, floor * sqrt(n)) over its n samples, the
+    floor being ``floor`` times the largest sampled plain value of the volume
+    (so that bricks whose plain values are all ~0 pass when the surrogate is
+    ~0 too).  A brick above ``tol`` -- or without a sample far enough from the
+    volume's border -- is recomputed with the plain kernels on the brick plus
+    the plain filter's full reach (exact), or, when that would cost more than
+    one plain pass over the volume, copied from one such pass."""
 
-    def __init__(self, model, shape, brick=64, tol=0.02, mode="reflect", alpha=0.5, beta=0.5):
+    def __init__(self, model, shape, brick=64, tol=0.02, mode="reflect", alpha=0.5, beta=0.5,
+                 floor=0.05):
         if model.gamma is None:
             raise ValueError("the surrogate needs a fixed gamma")
         self.model, self.shape = model, tuple(int(s) for s in shape)
         self.brick, self.tol, self.mode = int(brick), float(tol), mode
-        self.alpha, self.beta = float(alpha), float(beta)
+        self.alpha, self.beta, self.floor = float(alpha), float(beta), float(floor)
         comp = _composite_kernels(model.sigmas)
         kc, info, off = [], [], 0
-        self.reach = 0
         amax = rmax = 0
         for ks, L, A in comp:
             kc += list(ks)
@@ -398,33 +406,52 @@ class CheckedSurrogate:
         D, H, W = self.shape
         b = self.brick
         self.grid = tuple(-(-n // b) for n in self.shape)
-        lines = []
-        for bz in range(self.grid[0]):
-            for by in range(self.grid[1]):
-                for bx in range(self.grid[2]):
-                    z0, y0, x0 = bz * b, by * b, bx * b
-                    z1, y1, x1 = min(z0 + b, D), min(y0 + b, H), min(x0 + b, W)
-                    xc = min(max((x0 + x1) // 2, amax), W - 1 - rmax)
-                    yc = min(max((y0 + y1) // 2, amax), H - 1 - rmax)
-                    za, zb = max(z0, amax), min(z1, D - rmax, max(z0, amax) + 64)
-                    if not (x0 <= xc < x1 and y0 <= yc < y1) or zb <= za:
-                        za = zb = 0  # no exact sample inside this brick
-                    lines.append((xc, yc, za, zb))
+        self.nb = self.grid[0] * self.grid[1] * self.grid[2]
+        lines, owner = [], []
+        for i in range(self.nb):
+            bz, by, bx = i // (self.grid[1] * self.grid[2]), (i // self.grid[2]) % self.grid[1], \
+                i % self.grid[2]
+            z0, y0, x0 = bz * b, by * b, bx * b
+            z1, y1, x1 = min(z0 + b, D), min(y0 + b, H), min(x0 + b, W)
+            za, zb = max(z0, amax), min(z1, D - rmax, max(z0, amax) + 64)
+            seen = set()
+            for fy, fx in ((1, 1), (1, 3), (3, 1), (3, 3)):
+                xc = min(max(x0 + (x1 - x0) * fx // 4, amax), W - 1 - rmax)
+                yc = min(max(y0 + (y1 - y0) * fy // 4, amax), H - 1 - rmax)
+                if not (x0 <= xc < x1 and y0 <= yc < y1) or zb <= za or (xc, yc) in seen:
+                    continue
+                seen.add((xc, yc))
+                lines.append((xc, yc, za, zb))
+                owner.append(i)
+        if not lines:
+            lines, owner = [(0, 0, 0, 0)], [0]
+        self.owner = np.asarray(owner)
+        self.nl = len(lines)
         self.lines = torch.tensor(lines, dtype=torch.int32, device=_dev()).contiguous()
-        self.nb = len(lines)
-        self.plain_s = torch.empty(self.nb * 64, dtype=torch.float32, device=_dev())
-        self.rel = torch.empty(self.nb, dtype=torch.float32, device=_dev())
+        self.plain_s = torch.empty(self.nl * 64, dtype=torch.float32, device=_dev())
+        self.err = torch.empty(self.nl * 3, dtype=torch.float32, device=_dev())
         self._plans = {}
 
     def check(self, vol, sur):
+        """Per-brick sampled relative L2 error (NaN: no sample in the brick)."""
         lib = _lib.lib()
         _check(lib.hc_frangi_check(_array.hc_array(vol), _array.hc_array(sur),
-                                   self.lines.data_ptr(), self.nb, self.kc.data_ptr(),
+                                   self.lines.data_ptr(), self.nl, self.kc.data_ptr(),
                                    self.kinfo.data_ptr(), self.kinfo_host,
                                    len(self.model.sigmas), 0, self.alpha, self.beta,
                                    float(self.model.gamma), self.plain_s.data_ptr(),
-                                   self.rel.data_ptr(), _array.stream()), "frangi check")
-        return self.rel
+                                   self.err.data_ptr(), _array.stream()), "frangi check")
+        e = self.err.cpu().numpy().astype(np.float64).reshape(-1, 3)
+        ps = self.plain_s.cpu().numpy()
+        top = float(np.nanmax(ps)) if np.isfinite(ps).any() else 0.0
+        dn = np.bincount(self.owner, e[:, 0], minlength=self.nb)
+        nn = np.bincount(self.owner, e[:, 1], minlength=self.nb)
+        n = np.bincount(self.owner, e[:, 2], minlength=self.nb)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            den = np.maximum(np.sqrt(nn), self.floor * top * np.sqrt(n))
+            rel = np.where(n > 0, np.sqrt(dn) / np.where(den > 0, den, 1.0), np.nan)
+            rel = np.where((n > 0) & (den == 0), np.where(dn > 0, np.inf, 0.0), rel)
+        return rel
 
     def _plain(self, region):
         from .skimage.filters.ridges import FrangiPlan
@@ -445,7 +472,7 @@ class CheckedSurrogate:
         vol = _array.as_tensor(vol).contiguous()
         if sur is None:
             sur = self.model(vol, self.mode)
-        rel = self.check(vol, sur).cpu().numpy()
+        rel = self.check(vol, sur)
         if force is not None:
             bad = sorted(set(int(i) for i in force))
         else:
@@ -476,3 +503,48 @@ class CheckedSurrogate:
                 "sampled_rel_l2_median": float(np.median(finite)) if finite.size else None,
                 "bricks_without_samples": int(np.isnan(rel).sum())}
         return sur, info
+
+
+class SurrogateBatch3D:
+    """Surrogate Frangi over many uneven volumes as one launch sequence that
+    can be captured into a HIP graph: per volume the padding and the five
+    layers, work buffers sized for the largest volume and reused in stream
+    order."""
+
+    def __init__(self, model, volumes, mode="reflect", cval=0.0):
+        self.model, self.mode, self.cval = model, mode, float(cval)
+        self.volumes = [_array.as_tensor(v).contiguous() for v in volumes]
+        pads = [tuple(int(s) + 2 * HALO for s in v.shape) for v in self.volumes]
+        self.xbuf = torch.empty(max(d * h * w for d, h, w in pads), dtype=torch.bfloat16,
+                                device=_dev())
+        n1 = max((d - 2) * (h - 2) * (w - 2) * 24 for d, h, w in pads)
+        self.bufs = [torch.empty(n1, dtype=torch.bfloat16, device=_dev()) for _ in range(2)]
+        self.outputs = [torch.empty_like(v) for v in self.volumes]
+        self.voxels = sum(v.numel() for v in self.volumes)
+
+    def run_one(self, i):
+        from .skimage._vendored._ndimage_util import _mode_code
+
+        v = self.volumes[i]
+        D, H, W = (int(s) + 2 * HALO for s in v.shape)
+        x = self.xbuf[: D * H * W].view(D, H, W, 1)
+        _check(_lib.lib().hc_cnn3d_pad(_array.hc_array(v), _array.hc_array(x), HALO,
+                                       _mode_code(self.mode), self.cval, _array.stream()),
+               "pad")
+        self.model.forward(x, bufs=self.bufs, out=self.outputs[i])
+
+    def run(self):
+        for i in range(len(self.volumes)):
+            self.run_one(i)
+        return self.outputs
+
+    def capture(self):
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self.run()
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run()
+        return g

@@ -234,7 +234,7 @@ int hc_fill_le(const hc_array* a, double threshold, double value, void* stream);
  * reference; it stands in for filters/ridges.py:322-476 and is judged
  * against the plain kernels).  Valid 3x3x3 convolutions on channels-last
  * bfloat16 activations, bf16 weights, float32 accumulation on the matrix
- * cores.  act: 0 none, 1 GELU (erf), 2 clamp to [0, 1]. */
+ * cores.  act: 0 none, 1 GELU (tanh form), 2 clamp to [0, 1]. */
 
 /* bytes of the packed (matrix-core operand order) weights of a layer */
 int hc_cnn3d_packed_size(int ci, int co);
@@ -262,18 +262,18 @@ int hc_cnn3d_mse_grad(const float* y, const float* t, float* g, int64_t n, float
 int hc_cnn3d_adam(float* p, const float* g, float* m, float* v, int64_t n, double lr,
                   double b1, double b2, double eps, int step, void* stream);
 
-/* Sampled plain-Frangi check of the surrogate: for every brick, the plain 3-D
- * Frangi (filters/ridges.py:322-476 with a fixed gamma) evaluated exactly
- * on up to 64 voxels of one line along axis 0 (lines[4b..4b+3] = x0, y0,
- * za, zb), placed by the caller where no Gaussian pass reaches the volume's
+/* Sampled plain-Frangi check of the surrogate: for each of nb lines, the
+ * plain 3-D Frangi (filters/ridges.py:322-476 with a fixed gamma) evaluated
+ * exactly on up to 64 voxels along axis 0 (lines[4i..4i+3] = x0, y0, za,
+ * zb), placed by the caller where no Gaussian pass reaches the volume's
  * border; per sigma kc holds the composite kernels SS, SD, DD (full
  * convolutions of the pass kernels, L taps each, anchor A = 2 * anchor) and
- * kinfo (offset, L, A).  plain_out: nb x 64 values; rel_out: per brick the
- * relative L2 error of the surrogate output sur on the samples. */
+ * kinfo (offset, L, A).  plain_out: nb x 64 values; err_out: per line
+ * sum (sur - plain)^2, sum plain^2 and the number of samples. */
 int hc_frangi_check(const hc_array* vol, const hc_array* sur, const int32_t* lines_dev, int nb,
                     const float* kc_dev, const int32_t* kinfo_dev, const int32_t* kinfo_host,
                     int nsig, int black_ridges, double alpha, double beta, double gamma,
-                    float* plain_out, float* rel_out, void* stream);
+                    float* plain_out, float* err_out, void* stream);
 
 /* rgb2gray: color/colorconv.py:1037-1095 (+ _prepare_colorarray :160-180).
  * in: (..., 3) C-contiguous, any real dtype; out: (...) float32/float64. */

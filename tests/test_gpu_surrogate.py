@@ -66,7 +66,7 @@ def test_conv_layer(gpu, ci, co, act, f32, shape):
     np.testing.assert_allclose(pre.double().cpu().numpy(), ref.numpy(), rtol=0,
                                atol=1e-5 * max(scale, 1.0))
     if act == 1:
-        exp = Fnn.gelu(pre.double().cpu())
+        exp = Fnn.gelu(pre.double().cpu(), approximate="tanh")
     elif act == 2:
         exp = pre.double().cpu().clamp(0, 1)
     else:
@@ -75,9 +75,10 @@ def test_conv_layer(gpu, ci, co, act, f32, shape):
     if f32:
         np.testing.assert_allclose(got.numpy(), exp.numpy(), rtol=1e-6, atol=1e-6)
     else:
-        # float32 GELU then round-to-nearest bf16: one bf16 ulp at most
+        # float32 GELU (hardware exp / reciprocal, ~1e-6 relative) then
+        # round-to-nearest bf16: one bf16 ulp at most
         np.testing.assert_allclose(got.numpy(), exp.to(torch.bfloat16).double().numpy(),
-                                   rtol=2.0 ** -7, atol=1e-6)
+                                   rtol=2.0 ** -7, atol=1e-5)
 
 
 def test_backward_kernels(gpu):
@@ -118,11 +119,11 @@ def test_backward_kernels(gpu):
     gg = torch.randn(1000, generator=g).cuda()
     exp = gg.double().cpu()
     p = pre.double().cpu().requires_grad_(True)
-    Fnn.gelu(p).backward(exp)
+    Fnn.gelu(p, approximate="tanh").backward(exp)
     g2 = gg.clone()
     L.check(lib.hc_cnn3d_gelu_bwd(g2.data_ptr(), pre.data_ptr(), g2.numel(), st))
     torch.cuda.synchronize()
-    np.testing.assert_allclose(g2.double().cpu().numpy(), p.grad.numpy(), rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(g2.double().cpu().numpy(), p.grad.numpy(), rtol=1e-4, atol=1e-5)
     # mse
     yv, tv = torch.randn(5000, generator=g).cuda(), torch.randn(5000, generator=g).cuda()
     gm = torch.empty_like(yv)
@@ -172,15 +173,16 @@ def test_surrogate_forward_shape_and_training_descends(gpu):
 def _sample_positions(cs):
     lines = cs.lines.cpu().numpy()
     pos = []
-    for b, (x0, y0, za, zb) in enumerate(lines):
+    for li, (x0, y0, za, zb) in enumerate(lines):
         for k, z in enumerate(range(za, zb)):
-            pos.append((b, k, z, y0, x0))
+            pos.append((li, k, z, y0, x0))
     return pos
 
 
 def test_frangi_check_samples_match_plain(gpu):
     """The sampled exact plain values equal the plain Frangi (fused kernels,
-    same fixed gamma) at those voxels within float32 rounding."""
+    same fixed gamma) at those voxels within float32 rounding; the per-brick
+    error follows from the samples."""
     from cucim_amd import synthetic
     from cucim_amd.surrogate import CheckedSurrogate, FrangiSurrogate, frangi_label
 
@@ -188,27 +190,25 @@ def test_frangi_check_samples_match_plain(gpu):
     m = FrangiSurrogate(sigmas=(1, 2, 3, 4, 5), gamma=0.17, seed=2)
     cs = CheckedSurrogate(m, vol.shape, brick=64)
     sur = m(vol)
-    cs.check(vol, sur)
+    rel = cs.check(vol, sur)
     plain = frangi_label(vol, m.sigmas, m.gamma).cpu().numpy()
     samp = cs.plain_s.cpu().numpy().reshape(-1, 64)
     pos = _sample_positions(cs)
     assert len(pos) > 0
-    got = np.array([samp[b, k] for b, k, z, y, x in pos])
-    exp = np.array([plain[z, y, x] for b, k, z, y, x in pos])
+    got = np.array([samp[li, k] for li, k, z, y, x in pos])
+    exp = np.array([plain[z, y, x] for li, k, z, y, x in pos])
     np.testing.assert_allclose(got, exp, rtol=1e-3, atol=1e-5 * max(1.0, exp.max()))
-    # the per-brick error of the kernel matches the samples
-    rel = cs.rel.cpu().numpy()
     s = sur.cpu().numpy()
+    top = got.max()
     for b in range(cs.nb):
-        pb = [(k, z, y, x) for bb, k, z, y, x in pos if bb == b]
+        pb = [(z, y, x) for li, k, z, y, x in pos if cs.owner[li] == b]
         if not pb:
             assert np.isnan(rel[b])
             continue
-        e = np.array([plain[z, y, x] for _, z, y, x in pb])
-        q = np.array([s[z, y, x] for _, z, y, x in pb])
-        den = np.linalg.norm(e)
-        if den > 1e-6:
-            assert abs(rel[b] - np.linalg.norm(q - e) / den) <= 1e-2 * rel[b] + 1e-5
+        e = np.array([plain[p] for p in pb])
+        q = np.array([s[p] for p in pb])
+        den = max(np.linalg.norm(e), cs.floor * top * np.sqrt(len(pb)))
+        assert abs(rel[b] - np.linalg.norm(q - e) / den) <= 1e-2 * rel[b] + 1e-5
 
 
 @pytest.mark.parametrize("path", ["region", "mass"])
@@ -234,3 +234,21 @@ def test_fallback_is_plain(gpu, path):
               slice(bx * 64, bx * 64 + 64))
         ref = plain[sl] if i in force else raw[sl]
         np.testing.assert_array_equal(out[sl].view(np.uint32), ref.view(np.uint32))
+
+
+def test_surrogate_batch_graph_equals_eager(gpu):
+    from cucim_amd import synthetic
+    from cucim_amd.surrogate import FrangiSurrogate, SurrogateBatch3D
+
+    m = FrangiSurrogate(sigmas=(1, 2, 3), gamma=0.2, seed=7)
+    vols = [synthetic.tube_volume(s, ntubes=3, seed=i, noise=0.08)
+            for i, s in enumerate([(48, 70, 52), (90, 49, 61), (64, 64, 64)])]
+    sb = SurrogateBatch3D(m, vols)
+    g = sb.capture()
+    g.replay()
+    torch.cuda.synchronize()
+    got = [o.clone() for o in sb.outputs]
+    for v, o in zip(vols, got):
+        exp = m(v)
+        torch.cuda.synchronize()
+        assert torch.equal(o.view(torch.int32), exp.view(torch.int32))
