@@ -44,6 +44,8 @@ def _parse():
                    default="all",
                    help="restrict the run (profiler passes); 'copy' runs the plain copy "
                         "kernel that calibrates the HBM byte counters")
+    p.add_argument("--hessian-mfma", action="store_true",
+                   help="run the float32 Hessian passes on the matrix cores")
     p.add_argument("--no-check", action="store_true",
                    help="skip the fused-vs-reference-order cross-check")
     p.add_argument("--cpu-tiles", type=int, default=128,
@@ -526,6 +528,10 @@ def main():
     import torch
 
     torch.cuda.set_device(0)
+    if args.hessian_mfma:
+        from cucim_amd import _lib
+
+        _lib.check(_lib.lib().hc_tune_set(_lib.TUNE_HESSIAN_MFMA, 1))
     if args.only == "copy":
         print(json.dumps({"copy": bench_copy()}))
         return

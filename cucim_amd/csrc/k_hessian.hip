@@ -20,6 +20,7 @@
 // work) arrays, in place, because each work-group reads its lines completely
 // before writing the same lines.
 #include <algorithm>
+#include <type_traits>
 #include <cstring>
 #include <utility>
 #include <vector>
@@ -59,6 +60,10 @@ struct ChainArgs {
   int negate;   // first pass of Frangi with black_ridges=False: image = -image
   W cval;
   TapPack<W> S, D;  // same padded length and anchor
+  int t0, Km;       // matrix-core variant: taps t0 .. t0 + Km - 1 hold every
+                    // nonzero weight of S and D
+  int shift;        // matrix-core variant: buffer row of line position 0
+  int njobs;        // matrix-core variant: jobs (persistent grid)
 };
 
 // --------------------------------------------------- strided chain pass --
@@ -222,6 +227,300 @@ __global__ __launch_bounds__(256) void k_chain_strided(const ChainArgs<W> a) {
   });
 }
 
+// ------------------------------------- strided chain pass, matrix cores --
+// The same two passes on the matrix cores (float32): a pass along the line,
+// for 16 output positions p and the work-group's 16 lines c, is the product
+//   Y[p][c] = sum_q T[p][q] X[q][c],  T[p][q] = w[q - p]  (Toeplitz),
+// done with v_mfma_f32_16x16x4f32 over input blocks q = 16 (b + d) + 4 u + k:
+// A[i][k] = w[16 d + 4 u + k - i] (from a zero-padded LDS copy of the taps),
+// B[k][c] = X[q][c] (the LDS line buffer, conflict-free), D[i][c] -> lane l
+// holds positions 4 (l / 16) + r of line l % 16.  The sums are rounded as
+// the matrix core rounds them, not tap by tap; a work-group whose lines hold
+// a non-finite value computes every output instead as the reference does
+// (float32 fused multiply-add per non-zero tap, in tap order), in the same
+// register layout.
+//
+// Only the span t0 .. t0 + Km - 1 of taps that holds every nonzero weight of
+// both sets is used (Gaussian tails round to zero in float32).  Buffer row r
+// holds line position r - shift, shift = anchor - t0, so output p reads rows
+// p .. p + Km - 1.  A work-group computes kChainSPW consecutive stages (a
+// tile of 16 lines of one job) and keeps the next stage's input in flight
+// in registers while the current one is computed.
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4_t mfma4(float a, float b, f32x4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// The wave's index in the work-group, as a wave-uniform (scalar) value, so
+// that tests on it are scalar branches rather than lane masks.
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
+
+// Values per thread of one tile's interior (16 lines of N <= 16 NPF) held
+// in flight.
+template <int NBW>
+constexpr int mfma_npf() {
+  return NBW <= 4 ? 16 : 32;
+}
+
+// Blocks b = wave + 4 i (i < NBW) of 16 positions: acc[s][i] for the NS
+// weight sets wp[s] (zero-padded: trimmed tap t at wp[15 + t]).  The A
+// fragment of step (d, u) serves every block of the wave.
+template <int NBW, int NS>
+__device__ __forceinline__ void chain_pass_mfma(f32x4_t (&acc)[NS][NBW], const float* X,
+                                                const float* const (&wp)[NS], int nd,
+                                                int nblk, int wv, int lane) {
+  const int kk = lane >> 4, ii = lane & 15;
+#pragma unroll
+  for (int i = 0; i < NBW; ++i) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc[s][i] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  }
+  // blocks past the last are computed on the last one's rows (no branch in
+  // the loop); their results are never stored
+  int rb[NBW];
+#pragma unroll
+  for (int i = 0; i < NBW; ++i) rb[i] = (min(wv + 4 * i, nblk - 1) * 16 + kk) * kChS + ii;
+  for (int d = 0; d < nd; ++d) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int wi = 15 + 16 * d + 4 * u + kk - ii;
+      float av[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) av[s] = wp[s][wi];
+#pragma unroll
+      for (int i = 0; i < NBW; ++i) {
+        const float bx = X[rb[i] + (16 * d + 4 * u) * kChS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) acc[s][i] = mfma4(av[s], bx, acc[s][i]);
+      }
+    }
+  }
+}
+
+// The reference's arithmetic for the same outputs, in the same layout (the
+// taps outside the span are zero, so the same products in the same order).
+template <int NBW, int NS>
+__device__ __forceinline__ void chain_pass_exact(f32x4_t (&acc)[NS][NBW], const float* X,
+                                                 const float* const (&wp)[NS], int Km,
+                                                 int nblk, int wv, int lane) {
+  const int kk = lane >> 4, ii = lane & 15;
+#pragma unroll
+  for (int i = 0; i < NBW; ++i) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc[s][i] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    const int b = wv + 4 * i;
+    if (b >= nblk) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int p = 16 * b + 4 * kk + r;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        float v = 0.f;
+        for (int t = 0; t < Km; ++t) {
+          const float w = wp[s][15 + t];
+          if (w != 0.f) v = fmaf(X[(p + t) * kChS + ii], w, v);
+        }
+        acc[s][i][r] = v;
+      }
+    }
+  }
+}
+
+template <int NBW, int NS>
+__device__ __forceinline__ void chain_pass(bool exact, f32x4_t (&acc)[NS][NBW], const float* X,
+                                           const float* const (&wp)[NS], int Km, int nd,
+                                           int nblk, int wv, int lane) {
+  if (exact)
+    chain_pass_exact<NBW, NS>(acc, X, wp, Km, nblk, wv, lane);
+  else
+    chain_pass_mfma<NBW, NS>(acc, X, wp, nd, nblk, wv, lane);
+}
+
+__device__ __forceinline__ float load_in(const void* in, int dt, int64_t g, bool negate) {
+  const float y =
+      dt == HC_FLOAT32 ? reinterpret_cast<const float*>(in)[g] : load_as<float>(in, g, dt);
+  return negate ? -y : y;
+}
+
+// Zero-padded trimmed taps of both sets into LDS.
+__device__ __forceinline__ void stage_trimmed(float* wpS, float* wpD, const TapPack<float>& S,
+                                              const TapPack<float>& D, int t0, int Km, int WP,
+                                              int tid) {
+  for (int e = tid; e < WP; e += 256) {
+    const int t = e - 15;
+    wpS[e] = (t >= 0 && t < Km) ? S.w[t0 + t] : 0.f;
+    wpD[e] = (t >= 0 && t < Km) ? D.w[t0 + t] : 0.f;
+  }
+}
+
+// Boundary-extend rows of position outside [0, N) from the rows inside.
+__device__ __forceinline__ void fill_halo(float* X, float* Y, int E, int64_t N, int shift,
+                                          int mode, float cval, int tid) {
+  const int nh = E - (int)N;
+  for (int e = tid; e < nh * kChC; e += 256) {
+    const int h = e / kChC, c = e - h * kChC;
+    const int q = h < shift ? h : h + (int)N;
+    const int64_t r = remap((int64_t)q - shift, N, mode);
+    float vx = cval, vy = cval;
+    if (r >= 0) {
+      vx = X[(r + shift) * kChS + c];
+      if (Y) vy = Y[(r + shift) * kChS + c];
+    }
+    X[q * kChS + c] = vx;
+    if (Y) Y[q * kChS + c] = vy;
+  }
+}
+
+// Stages (tile, job) per work-group of the matrix-core chain pass.
+constexpr int kChainSPW = 2;
+
+template <int NBW>
+__global__ __launch_bounds__(256) void k_chain_mfma(const ChainArgs<float> a) {
+  constexpr int NPF = mfma_npf<NBW>();
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int E = a.E, Nr = a.Nr, Km = a.Km, shift = a.shift;
+  const int nd = (Km + 30) / 16;  // input blocks per output block
+  const int WP = 16 * nd + 16;    // padded tap array length
+  float* bufA = reinterpret_cast<float*>(smem_raw);
+  float* bufB = bufA + E * kChS;
+  float* wpS = bufB + E * kChS;
+  float* wpD = wpS + WP;
+  const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
+  const int kk = lane >> 4, ii = lane & 15;
+  stage_trimmed(wpS, wpD, a.S, a.D, a.t0, Km, WP, tid);
+  const int64_t N = a.N, inner = a.inner, nlines = a.nlines;
+  const int mode = a.mode, nblk = Nr / 16;
+  const int64_t ntile = (nlines + kChC - 1) / kChC, nst = ntile * a.njobs;
+  const int64_t G = gridDim.x;
+  // this thread stages line c = tid % 16, positions tid / 16 + 16 u
+  const int cs = tid & 15, ns0 = tid >> 4;
+  auto load = [&](float (&v)[NPF], int64_t st) {
+    const int job = (int)(st / ntile);
+    const int64_t L = (st - job * ntile) * kChC + cs;
+    const ChainJob& jb = a.job[job];
+    const bool ok = L < nlines;
+    const int64_t lo = ok ? L / inner : 0, li = ok ? L - lo * inner : 0;
+    const int64_t base = lo * N * inner + li;
+#pragma unroll
+    for (int u = 0; u < NPF; ++u) {
+      const int n = ns0 + 16 * u;
+      v[u] = (ok && n < N) ? load_in(jb.in, jb.in_dt, base + n * inner, a.negate != 0) : 0.f;
+    }
+  };
+  const bool cval_bad = !finite(a.cval);
+  float pf[NPF];
+  const int64_t st0 = xcd_swizzle(blockIdx.x, G) * kChainSPW;
+  load(pf, st0);
+#pragma unroll
+  for (int k = 0; k < kChainSPW; ++k) {
+    const int64_t st = st0 + k;
+    if (st >= nst) break;
+    const int job = (int)(st / ntile);
+    const int64_t L0 = (st - job * ntile) * kChC;
+    const ChainJob& jb = a.job[job];
+    bool bad = false;
+#pragma unroll
+    for (int u = 0; u < NPF; ++u) {
+      const int n = ns0 + 16 * u;
+      if (n < N) {
+        bufA[(shift + n) * kChS + cs] = pf[u];
+        bad |= !finite(pf[u]);
+      }
+    }
+    const bool skip0 = __syncthreads_or(bad) || cval_bad;
+    // the work-group's next stage is in flight during this one's passes
+    if (st + 1 < nst && st + 1 < st0 + kChainSPW) load(pf, st + 1);
+    fill_halo(bufA, nullptr, E, N, shift, mode, a.cval, tid);
+    __syncthreads();
+    bool need_s = false, need_d = false;
+    for (int k = 0; k < jb.nout; ++k) {
+      need_s |= jb.o[k].first == 0;
+      need_d |= jb.o[k].first == 1;
+    }
+    // first passes into registers
+    f32x4_t f[2][NBW];
+    if (need_s && need_d) {
+      const float* const wp[2] = {wpS, wpD};
+      chain_pass<NBW, 2>(skip0, f, bufA, wp, Km, nd, nblk, wv, lane);
+    } else {
+      f32x4_t g[1][NBW];
+      const float* const wp[1] = {need_s ? wpS : wpD};
+      chain_pass<NBW, 1>(skip0, g, bufA, wp, Km, nd, nblk, wv, lane);
+#pragma unroll
+      for (int i = 0; i < NBW; ++i) {
+        f[0][i] = g[0][i];
+        f[1][i] = g[0][i];
+      }
+    }
+    __syncthreads();  // every read of the staged input is done
+    bool bad1 = false;
+#pragma unroll
+    for (int i = 0; i < NBW; ++i) {
+      const int b = wv + 4 * i;
+      if (b >= nblk) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int q = shift + 16 * b + 4 * kk + r;
+        bufA[q * kChS + ii] = f[0][i][r];
+        bufB[q * kChS + ii] = f[1][i][r];
+        bad1 |= !(finite(f[0][i][r]) && finite(f[1][i][r]));
+      }
+    }
+    const bool skip1 = __syncthreads_or(bad1) || skip0;
+    fill_halo(bufA, bufB, E, N, shift, mode, a.cval, tid);
+    __syncthreads();
+    // second passes, per source buffer
+    const int64_t L = L0 + ii;
+    const bool lok = L < nlines;
+    const int64_t lo = lok ? L / inner : 0, li = lok ? L - lo * inner : 0;
+    const int64_t base = lo * N * inner + li;
+#pragma unroll 1
+    for (int src = 0; src < 2; ++src) {
+      int ks = -1, kd = -1;
+      for (int k = 0; k < jb.nout; ++k) {
+        if (jb.o[k].first != src) continue;
+        if (jb.o[k].second == 0) ks = k; else kd = k;
+      }
+      if (ks < 0 && kd < 0) continue;
+      const float* X = src ? bufB : bufA;
+      f32x4_t y[2][NBW];
+      if (ks >= 0 && kd >= 0) {
+        const float* const wp[2] = {wpS, wpD};
+        chain_pass<NBW, 2>(skip1, y, X, wp, Km, nd, nblk, wv, lane);
+      } else {
+        f32x4_t g[1][NBW];
+        const float* const wp[1] = {ks >= 0 ? wpS : wpD};
+        chain_pass<NBW, 1>(skip1, g, X, wp, Km, nd, nblk, wv, lane);
+#pragma unroll
+        for (int i = 0; i < NBW; ++i) {
+          y[0][i] = g[0][i];
+          y[1][i] = g[0][i];
+        }
+      }
+      float* os = ks >= 0 ? reinterpret_cast<float*>(jb.o[ks].out) : nullptr;
+      float* od = kd >= 0 ? reinterpret_cast<float*>(jb.o[kd].out) : nullptr;
+      if (!lok) continue;
+#pragma unroll
+      for (int i = 0; i < NBW; ++i) {
+        const int b = wv + 4 * i;
+        if (b >= nblk) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t n = 16 * b + 4 * kk + r;
+          if (n < N) {
+            if (os) os[base + n * inner] = y[0][i][r];
+            if (od) od[base + n * inner] = y[1][i][r];
+          }
+        }
+      }
+    }
+    __syncthreads();  // the buffers are restaged for the next stage
+  }
+}
+
 // ------------------------------------------------ contiguous chain pass --
 // The first Hessian pass when it runs along a contiguous axis (2-D arrays
 // whose axis-0 lines are too long for the strided pass).  Lines are rows.
@@ -348,6 +647,8 @@ struct FinalArgs {
   double alpha_sq, beta_sq, gamma_sq;
   const double* gamma_sq_dev;  // when non-null, overrides gamma_sq
   void* rg_max;     // uint32 (float) / uint64 (double) bit pattern
+  int t0, Km, shift;  // matrix-core variant: nonzero tap span of S and D,
+                      // buffer row of line position 0
 };
 
 template <typename W>
@@ -520,6 +821,191 @@ __global__ __launch_bounds__(256) void k_final_contig(const FinalArgs<W> a) {
   if (a.epi == EPI_FRANGI && a.store_eig) atomic_max_bits<W>(a.rg_max, rgmax);
 }
 
+// The final pass on the matrix cores (float32): tiles of 16 rows, each row
+// staged whole and transposed (position-major, row-minor) so that the
+// Toeplitz passes of the strided chain apply unchanged; every lane ends up
+// with all Hessian elements of 4 NBW voxels of one row, and the epilogue is
+// the one of k_final_contig.  The tile's next input is in flight in
+// registers while the current one is computed.
+struct FinalGeom {
+  int E, Km, shift, nd, nblk;
+  int64_t N, nrows, ntile, G;
+};
+
+// Thread's share of a tile's interior: row c = u % 16, position
+// tid + 256 (u / 16).
+template <int NPF>
+__device__ __forceinline__ void final_load(const FinalArgs<float>& a, const FinalGeom& g,
+                                           float (&v)[NPF], int J, int64_t tile, int tid) {
+  const int64_t row0 = tile * kChC;
+  const int n = (int)g.N;
+#pragma unroll
+  for (int u = 0; u < NPF; ++u) {
+    // a wave-uniform row pointer and a 32-bit lane offset
+    const int64_t row = row0 + (u & 15);
+    const float* rp = a.in[J] + row * g.N;
+    const int q = tid + 256 * (u >> 4);
+    v[u] = (row < g.nrows && q < n) ? rp[q] : 0.f;
+  }
+}
+
+// Input J of the tile: store the loaded input, start the next load, first
+// op, re-extend, second op into res.
+template <int NIN, int NBW, int J>
+__device__ __forceinline__ void final_input_mfma(const FinalArgs<float>& a, const FinalGeom& g,
+                                                 float* xin, float* xf, const float* wpS,
+                                                 const float* wpD, int64_t tile,
+                                                 float (&pf)[mfma_npf<NBW>()],
+                                                 f32x4_t (&res)[NBW]) {
+  const int E = g.E, shift = g.shift;
+  const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
+  const int kk = lane >> 4, ii = lane & 15;
+  constexpr int NPF = mfma_npf<NBW>();
+  bool bad = false;
+#pragma unroll
+  for (int u = 0; u < NPF; ++u) {
+    const int q = tid + 256 * (u >> 4);
+    if (q < g.N) {
+      xin[(shift + q) * kChS + (u & 15)] = pf[u];
+      bad |= !finite(pf[u]);
+    }
+  }
+  const bool skip0 = __syncthreads_or(bad) || !finite(a.cval);
+  if (J + 1 < NIN) final_load<NPF>(a, g, pf, J + 1 < NIN ? J + 1 : 0, tile, tid);
+  fill_halo(xin, nullptr, E, g.N, shift, a.mode, a.cval, tid);
+  __syncthreads();
+  f32x4_t f[1][NBW];
+  {
+    const float* const wp[1] = {a.first[J] ? wpD : wpS};
+    chain_pass<NBW, 1>(skip0, f, xin, wp, g.Km, g.nd, g.nblk, wv, lane);
+  }
+  bool bad1 = false;
+#pragma unroll
+  for (int i = 0; i < NBW; ++i) {
+    if (wv + 4 * i >= g.nblk) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      xf[(shift + 16 * (wv + 4 * i) + 4 * kk + r) * kChS + ii] = f[0][i][r];
+      bad1 |= !finite(f[0][i][r]);
+    }
+  }
+  const bool skip1 = __syncthreads_or(bad1) || skip0;
+  fill_halo(xf, nullptr, E, g.N, shift, a.mode, a.cval, tid);
+  __syncthreads();
+  {
+    const float* const wp[1] = {a.second[J] ? wpD : wpS};
+    chain_pass<NBW, 1>(skip1, f, xf, wp, g.Km, g.nd, g.nblk, wv, lane);
+  }
+#pragma unroll
+  for (int i = 0; i < NBW; ++i) res[i] = f[0][i];
+  // xin is next written after the barrier at the start of the next input,
+  // which every wave reaches only after its first pass here; xf is next
+  // written after the first pass of the next input, behind that barrier too
+}
+
+template <int NIN, int NBW, size_t... Js>
+__device__ __forceinline__ void final_all_mfma(const FinalArgs<float>& a, const FinalGeom& g,
+                                               float* xin, float* xf, const float* wpS,
+                                               const float* wpD, int64_t tile,
+                                               float (&pf)[mfma_npf<NBW>()],
+                                               f32x4_t (&res)[NIN][NBW],
+                                               std::index_sequence<Js...>) {
+  (final_input_mfma<NIN, NBW, (int)Js>(a, g, xin, xf, wpS, wpD, tile, pf, res[Js]), ...);
+}
+
+template <int NIN, int NBW>
+__global__ __launch_bounds__(256) void k_final_mfma(
+    const FinalArgs<float> a) {
+  constexpr int NPF = mfma_npf<NBW>();
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  FinalGeom g;
+  g.E = a.E;
+  g.Km = a.Km;
+  g.shift = a.shift;
+  g.nd = (a.Km + 30) / 16;
+  g.nblk = a.Nr / 16;
+  g.N = a.N;
+  g.nrows = a.nrows;
+  g.ntile = (a.nrows + kChC - 1) / kChC;
+  g.G = gridDim.x;
+  const int WP = 16 * g.nd + 16;
+  float* xin = reinterpret_cast<float*>(smem_raw);
+  float* xf = xin + g.E * kChS;
+  float* wpS = xf + g.E * kChS;
+  float* wpD = wpS + WP;
+  const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
+  const int kk = lane >> 4, ii = lane & 15;
+  stage_trimmed(wpS, wpD, a.S, a.D, a.t0, a.Km, WP, tid);
+  double gsq = a.gamma_sq;
+  if (a.epi == EPI_FRANGI && a.gamma_sq_dev) gsq = *a.gamma_sq_dev;
+  const int64_t N = g.N;
+  float rgmax = 0.f;
+  float pf[NPF];
+  const int64_t tile = xcd_swizzle(blockIdx.x, g.G);
+  final_load<NPF>(a, g, pf, 0, tile, tid);
+  {
+    f32x4_t res[NIN][NBW];
+    final_all_mfma<NIN, NBW>(a, g, xin, xf, wpS, wpD, tile, pf, res,
+                             std::make_index_sequence<NIN>{});
+    // epilogue (k_final_contig's)
+    const int64_t row = tile * kChC + ii;
+    if (row < g.nrows) {
+      // one voxel per iteration (a rolled loop: the eigen solves of all
+      // the lane's voxels would not fit in registers beside the tile's);
+      // the voxel's elements are picked from the registers by selects
+#pragma unroll 1
+      for (int v = 0; v < 4 * NBW; ++v) {
+        const int i = v >> 2;
+        if (wv + 4 * i >= g.nblk) break;
+        const int64_t p = 16 * (wv + 4 * i) + 4 * kk + (v & 3);
+        if (p >= N) continue;
+        float h[NIN];
+#pragma unroll
+        for (int j = 0; j < NIN; ++j) {
+          float x = res[j][0][0];
+#pragma unroll
+          for (int w = 1; w < 4 * NBW; ++w) x = v == w ? res[j][w >> 2][w & 3] : x;
+          h[j] = x;
+        }
+        const int64_t idx = row * N + p;
+        if (a.epi == EPI_STORE) {
+#pragma unroll
+          for (int j = 0; j < NIN; ++j) a.out[j][idx] = h[j];
+          continue;
+        }
+        if (NIN == 6) {
+          float l1, l2, l3;
+          // H order (H00, H01, H02, H11, H12, H22) = (a, d, f, b, e, c)
+          eig33<float>(h[0], h[3 % NIN], h[5 % NIN], h[1], h[4 % NIN], h[2], 0, 1, l1, l2, l3);
+          const float rg = frangi_rg3<float>(l1, l2, l3);
+          if (a.store_eig) {
+            a.eig[0][idx] = l1;
+            a.eig[1][idx] = l2;
+            a.eig[2][idx] = l3;
+            rgmax = nanmax<float>(rgmax, rg);
+          } else {
+            const float vv = frangi_v3<float>(l1, l2, l3, rg, a.alpha_sq, a.beta_sq, gsq);
+            a.fout[idx] = a.first_sigma ? vv : nanmax<float>(a.fout[idx], vv);
+          }
+        } else {
+          float l1, l2;
+          eig22<float>(h[0], h[1], h[2], 0, 1, l1, l2);
+          const float rg = frangi_rg2<float>(l1, l2);
+          if (a.store_eig) {
+            a.eig[0][idx] = l1;
+            a.eig[1][idx] = l2;
+            rgmax = nanmax<float>(rgmax, rg);
+          } else {
+            const float vv = frangi_v2<float>(l1, l2, rg, a.beta_sq, gsq);
+            a.fout[idx] = a.first_sigma ? vv : nanmax<float>(a.fout[idx], vv);
+          }
+        }
+      }
+    }
+  }
+  if (a.epi == EPI_FRANGI && a.store_eig) atomic_max_bits<float>(a.rg_max, rgmax);
+}
+
 // ---------------------------------------------------- small kernels --
 __global__ void k_frangi_reset(unsigned long long* rg) {
   if (threadIdx.x == 0 && blockIdx.x == 0) rg[0] = 0ull;
@@ -666,6 +1152,71 @@ int axis_taps(const hc_taps* ts, const hc_taps* td, AxisTaps<W>* out) {
   return HC_OK;
 }
 
+// Matrix-core passes: off unless the HC_TUNE_HESSIAN_MFMA knob is set (they
+// measure slower than the vector-ALU passes on the 256^3 workload).
+inline bool chain_mfma_enabled() { return tune_get(HC_TUNE_HESSIAN_MFMA) == 1; }
+
+// Geometry of the matrix-core passes over lines of N: the nonzero span of
+// both tap sets (zero taps at the ends add exact zeros for finite inputs;
+// non-finite tiles take the tap-by-tap path), the buffer shift and length,
+// the LDS bytes.  False when they do not apply.
+struct MfmaGeomHost {
+  int t0, Km, shift, Nr, E, nd, lds;
+};
+
+inline bool mfma_geom(const TapPack<float>& S, const TapPack<float>& D, int64_t N, int npf,
+                      MfmaGeomHost& g) {
+  if (!chain_mfma_enabled() || N < 1 || N > 512) return false;
+  int lo = S.K, hi = -1;
+  for (int t = 0; t < S.K; ++t)
+    if (S.w[t] != 0.f || D.w[t] != 0.f) {
+      lo = std::min(lo, t);
+      hi = t;
+    }
+  if (hi < 0) return false;
+  g.t0 = lo;
+  g.Km = hi - lo + 1;
+  g.shift = S.anchor - lo;
+  if (g.shift < 0 || g.shift > g.Km + 14) return false;
+  g.Nr = (int)((N + 15) / 16 * 16);
+  // the last input block reads row Nr + 16 nd - 17 <= Nr + Km + 13
+  g.E = g.Nr + g.Km + 14;
+  g.nd = (g.Km + 30) / 16;
+  g.lds = (2 * g.E * kChS + 2 * (16 * g.nd + 16)) * (int)sizeof(float);
+  return g.lds <= kMaxLds && N <= 16 * npf;
+}
+
+template <int NBW>
+int launch_chain_mfma_nbw(ChainArgs<float>& a, const MfmaGeomHost& g, hipStream_t st) {
+  const void* f = (const void*)k_chain_mfma<NBW>;
+  if (g.lds > 64 * 1024) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, g.lds);
+    if (e != hipSuccess) return hip_check(e, "hipFuncSetAttribute(k_chain_mfma)");
+  }
+  const int64_t work = (a.nlines + kChC - 1) / kChC * a.njobs;
+  const int64_t G = (work + kChainSPW - 1) / kChainSPW;
+  HC_LAUNCH("k_chain_mfma", st, (k_chain_mfma<NBW>), dim3((unsigned)G), dim3(256), g.lds, a);
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+// Matrix-core strided chain pass (float32, lines of at most 256: longer
+// ones would need more registers than a lane has); returns -1 when it does
+// not apply.
+inline int launch_chain_mfma(ChainArgs<float>& a, hipStream_t st) {
+  const int nblk = (int)((a.N + 15) / 16);
+  MfmaGeomHost g;
+  if (nblk > 16 || !mfma_geom(a.S, a.D, a.N, mfma_npf<4>(), g)) return -1;
+  a.t0 = g.t0;
+  a.Km = g.Km;
+  a.shift = g.shift;
+  a.Nr = g.Nr;
+  a.E = g.E;
+  if (nblk <= 4) return launch_chain_mfma_nbw<1>(a, g, st);
+  if (nblk <= 8) return launch_chain_mfma_nbw<2>(a, g, st);
+  return launch_chain_mfma_nbw<4>(a, g, st);
+}
+
 template <typename W>
 int run_chain_pass(const ChainJob* jobs, int njobs, const hc_array* shape_src, int axis,
                    const AxisTaps<W>& tp, int mode, double cval, int negate, hipStream_t st) {
@@ -680,6 +1231,13 @@ int run_chain_pass(const ChainJob* jobs, int njobs, const hc_array* shape_src, i
   a.cval = (W)cval;
   a.S = tp.S;
   a.D = tp.D;
+  if (ln.inner > 1 && std::is_same<W, float>::value) {
+    ChainArgs<float>& af = *reinterpret_cast<ChainArgs<float>*>(&a);
+    af.nlines = ln.outer * ln.inner;
+    af.njobs = njobs;
+    const int rc = launch_chain_mfma(af, st);
+    if (rc >= 0) return rc;
+  }
   if (ln.inner > 1) {
     a.nlines = ln.outer * ln.inner;
     a.Nr = (int)((ln.N + kChP - 1) / kChP * kChP);
@@ -731,8 +1289,49 @@ int prepare_final(FinalArgs<W>* a, const hc_array* shape_src, const AxisTaps<W>&
   return HC_OK;
 }
 
+template <int NIN, int NBW>
+int launch_final_mfma_nbw(const FinalArgs<float>& a, const MfmaGeomHost& g, hipStream_t st) {
+  const void* f = (const void*)k_final_mfma<NIN, NBW>;
+  if (g.lds > 64 * 1024) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, g.lds);
+    if (e != hipSuccess) return hip_check(e, "hipFuncSetAttribute(k_final_mfma)");
+  }
+  const int64_t G = (a.nrows + kChC - 1) / kChC;
+  HC_LAUNCH("k_final_mfma", st, (k_final_mfma<NIN, NBW>), dim3((unsigned)G), dim3(256), g.lds, a);
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+// Matrix-core final pass for float32 rows of at most 256 voxels (3-D: six
+// Hessian elements of 16 voxels per lane) or 512 (2-D); returns -1 when it
+// does not apply.
+template <int NIN>
+int launch_final_mfma(const FinalArgs<float>& a0, hipStream_t st) {
+  constexpr int kMaxN = NIN == 6 ? 256 : 512;
+  if (a0.N > kMaxN) return -1;
+  const int nblk = (int)((a0.N + 15) / 16);
+  MfmaGeomHost g;
+  if (!mfma_geom(a0.S, a0.D, a0.N, nblk <= 16 ? mfma_npf<4>() : mfma_npf<8>(), g)) return -1;
+  FinalArgs<float> a = a0;
+  a.t0 = g.t0;
+  a.Km = g.Km;
+  a.shift = g.shift;
+  a.Nr = g.Nr;
+  a.E = g.E;
+  constexpr int kTop = NIN == 6 ? 4 : 8;
+  if (nblk <= 4) return launch_final_mfma_nbw<NIN, 1>(a, g, st);
+  if (nblk <= 8) return launch_final_mfma_nbw<NIN, 2>(a, g, st);
+  if (nblk <= 16) return launch_final_mfma_nbw<NIN, 4>(a, g, st);
+  return launch_final_mfma_nbw<NIN, kTop>(a, g, st);
+}
+
 template <typename W>
 int launch_final(const FinalArgs<W>& a, hipStream_t st) {
+  if (std::is_same<W, float>::value) {
+    const FinalArgs<float>& af = *reinterpret_cast<const FinalArgs<float>*>(&a);
+    const int rc = a.nin == 6 ? launch_final_mfma<6>(af, st) : launch_final_mfma<3>(af, st);
+    if (rc >= 0) return rc;
+  }
   const int lds = (2 * a.rows * a.E + 2 * (a.S.K + 4)) * (int)sizeof(W);
   const int64_t blocks = (a.nrows + a.rows - 1) / a.rows;
   if (a.nin == 6)

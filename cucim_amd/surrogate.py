@@ -381,13 +381,15 @@ class CheckedSurrogate:
     (so that bricks whose plain values are all ~0 pass when the surrogate is
     ~0 too).  A brick above ``tol`` -- or without a sample far enough from the
     volume's border -- is recomputed with the plain kernels on the brick plus
-    the plain filter's full reach (exact), or, when that would cost more than
+    the plain filter's full reach rounded up to 16 voxels (exact), or, when that would cost more than
     one plain pass over the volume, copied from one such pass."""
 
     def __init__(self, model, shape, brick=64, tol=0.02, mode="reflect", alpha=0.5, beta=0.5,
                  floor=0.05):
         if model.gamma is None:
             raise ValueError("the surrogate needs a fixed gamma")
+        if int(brick) <= 0 or int(brick) % 16:
+            raise ValueError("brick must be a positive multiple of 16")
         self.model, self.shape = model, tuple(int(s) for s in shape)
         self.brick, self.tol, self.mode = int(brick), float(tol), mode
         self.alpha, self.beta, self.floor = float(alpha), float(beta), float(floor)
@@ -400,6 +402,11 @@ class CheckedSurrogate:
             off += 3 * L
             amax, rmax = max(amax, A), max(rmax, L - 1 - A)
         self.reach = max(amax, rmax)
+        # region halo: the reach rounded up to 16 voxels, so that a region
+        # starts at a multiple of 16 along every axis (the matrix-core line
+        # passes group a line's outputs in blocks of 16 from its start, and
+        # the grouping fixes the rounding)
+        self.halo = -(-self.reach // 16) * 16
         self.kc = torch.tensor(np.concatenate(kc).astype(np.float32), device=_dev())
         self.kinfo_host = (ctypes.c_int32 * len(info))(*info)
         self.kinfo = torch.tensor(info, dtype=torch.int32, device=_dev())
@@ -466,6 +473,19 @@ class CheckedSurrogate:
                         gamma=self.model.gamma, black_ridges=False), "plain frangi")
         return out
 
+    def _region(self, i):
+        """Brick i's bounds and those of its recompute region (the brick plus
+        the halo, clipped to the volume)."""
+        D, H, W = self.shape
+        b, r = self.brick, self.halo
+        gy, gx = self.grid[1], self.grid[2]
+        bz, by, bx = i // (gy * gx), (i // gx) % gy, i % gx
+        z0, y0, x0 = bz * b, by * b, bx * b
+        z1, y1, x1 = min(z0 + b, D), min(y0 + b, H), min(x0 + b, W)
+        return ((z0, z1, y0, y1, x0, x1),
+                (max(0, z0 - r), min(D, z1 + r), max(0, y0 - r), min(H, y1 + r),
+                 max(0, x0 - r), min(W, x1 + r)))
+
     def __call__(self, vol, sur=None, force=None):
         """Returns (output, info).  sur: a precomputed surrogate output;
         force: brick indices to recompute regardless of the check."""
@@ -478,20 +498,15 @@ class CheckedSurrogate:
         else:
             bad = [i for i in range(self.nb) if not (rel[i] <= self.tol)]
         D, H, W = self.shape
-        b, r = self.brick, self.reach
-        region_vox = min(D, b + 2 * r) * min(H, b + 2 * r) * min(W, b + 2 * r)
-        mass = len(bad) * region_vox > D * H * W
+        regions = [self._region(i) for i in bad]
+        region_vox = sum((rz1 - rz0) * (ry1 - ry0) * (rx1 - rx0)
+                         for _, (rz0, rz1, ry0, ry1, rx0, rx1) in regions)
+        mass = region_vox > D * H * W
         full = self._plain(vol) if (bad and mass) else None
-        gz, gy, gx = self.grid
-        for i in bad:
-            bz, by, bx = i // (gy * gx), (i // gx) % gy, i % gx
-            z0, y0, x0 = bz * b, by * b, bx * b
-            z1, y1, x1 = min(z0 + b, D), min(y0 + b, H), min(x0 + b, W)
+        for (z0, z1, y0, y1, x0, x1), (rz0, rz1, ry0, ry1, rx0, rx1) in regions:
             if full is not None:
                 src = full[z0:z1, y0:y1, x0:x1]
             else:
-                rz0, ry0, rx0 = max(0, z0 - r), max(0, y0 - r), max(0, x0 - r)
-                rz1, ry1, rx1 = min(D, z1 + r), min(H, y1 + r), min(W, x1 + r)
                 reg_in = torch.empty((rz1 - rz0, ry1 - ry0, rx1 - rx0), dtype=vol.dtype,
                                      device=vol.device)
                 _array.copy_cast(vol[rz0:rz1, ry0:ry1, rx0:rx1], reg_in)
