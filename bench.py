@@ -44,8 +44,8 @@ def _parse():
                    default="all",
                    help="restrict the run (profiler passes); 'copy' runs the plain copy "
                         "kernel that calibrates the HBM byte counters")
-    p.add_argument("--hessian-mfma", action="store_true",
-                   help="run the float32 Hessian passes on the matrix cores")
+    p.add_argument("--hessian-valu", action="store_true",
+                   help="run the fused Hessian passes on the vector ALUs only")
     p.add_argument("--no-check", action="store_true",
                    help="skip the fused-vs-reference-order cross-check")
     p.add_argument("--cpu-tiles", type=int, default=128,
@@ -73,11 +73,11 @@ def _events_time(fn, reps):
 def frangi_flops_per_voxel(plan):
     """Algorithmic FLOPs per voxel of the fused schedule: per sigma
     27 taps-sweeps (axis 0: 5, axis 1: 10, axis 2: 12 one-kernel sweeps of
-    the padded kernel length K), 2 FLOPs per tap."""
+    the kernel's K taps), 2 FLOPs per tap."""
     total = 0
     nd = plan.ndim
     for s in range(len(plan.sigmas)):
-        K = [((plan.ts[s * nd + x].K + 3) // 4) * 4 for x in range(nd)]
+        K = [plan.ts[s * nd + x].K for x in range(nd)]
         total += 2 * (5 * K[0] + 10 * K[1] + 12 * K[2])
     return total
 
@@ -528,10 +528,10 @@ def main():
     import torch
 
     torch.cuda.set_device(0)
-    if args.hessian_mfma:
+    if args.hessian_valu:
         from cucim_amd import _lib
 
-        _lib.check(_lib.lib().hc_tune_set(_lib.TUNE_HESSIAN_MFMA, 1))
+        _lib.check(_lib.lib().hc_tune_set(_lib.TUNE_HESSIAN_VALU, 1))
     if args.only == "copy":
         print(json.dumps({"copy": bench_copy()}))
         return
@@ -558,16 +558,15 @@ def main():
     flops_vox = frangi_flops_per_voxel(plan)
     # per-launch algorithmic FLOPs of the dominant kernel
     nd = 3
-    per_pass = {"k_chain_strided": None, "k_final_contig": None}
-    Ks = [[((plan.ts[s * nd + x].K + 3) // 4) * 4 for x in range(nd)]
-          for s in range(len(plan.sigmas))]
-    if dom == "k_final_contig":
+    Ks = [[plan.ts[s * nd + x].K for x in range(nd)] for s in range(len(plan.sigmas))]
+    if dom in ("k_final_contig", "k_final_mfma"):
         fl = sum(2 * 12 * k[2] for k in Ks) * nvox / len(Ks)
-    elif dom == "k_chain_strided":
+    elif dom in ("k_chain_strided", "k_chain_mfma"):
         fl = sum(2 * (5 * k[0] + 10 * k[1]) for k in Ks) * nvox / (2 * len(Ks))
     else:
         fl = 0.0
     achieved = fl / (avg_ms * 1e-3) / 1e12 if fl else None
+    on_mfma = dom.endswith("_mfma")
     corr = None
     if args.only == "all":
         corr = bench_correlate(tuple(int(v) for v in args.corr_sizes.split(",")))
@@ -597,16 +596,21 @@ def main():
                                "black_ridges=False, reflect",
                    "voxels": nvox},
         "roofline": {
-            "bound": "valu",
+            "bound": "mfma" if on_mfma else "valu",
             "kernel": dom,
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": (achieved / FP32_PEAK_TFLOPS) if achieved else None,
             "traffic": read_traffic(dom, "frangi_256cubed_counters.txt"),
-            "note": "FP32 FMA sweeps of the separable passes; packed FP32 is disabled in "
-                    "this build, so the reachable VALU ceiling is "
-                    f"{FP32_VALU_NOPACK_TFLOPS} TFLOP/s",
+            "note": ("separable passes as Toeplitz products on v_mfma_f32_16x16x4f32 "
+                     "(dense FP32 matrix peak); achieved counts the K real taps per "
+                     "output, the matrix cores also multiply the zero band of each "
+                     "16-output block (4*ceil((K+15)/4) products per output)")
+                    if on_mfma else
+                    ("FP32 FMA sweeps of the separable passes; packed FP32 is disabled in "
+                     "this build, so the reachable VALU ceiling is "
+                     f"{FP32_VALU_NOPACK_TFLOPS} TFLOP/s"),
             "avg_launch_ms": avg_ms,
             "launches_per_step": cnt / args.steps,
         },

@@ -26,7 +26,7 @@ MODE_REFLECT, MODE_MIRROR, MODE_NEAREST, MODE_WRAP, MODE_CONSTANT = range(5)
 ALGO_AUTO, ALGO_TILED, ALGO_GATHER = range(3)
 
 # hc_tune_knob
-TUNE_CONTIG_SEG, TUNE_SEP2D_TY, TUNE_HESSIAN_MFMA = 0, 1, 2
+TUNE_CONTIG_SEG, TUNE_SEP2D_TY, TUNE_HESSIAN_VALU = 0, 1, 2
 
 # hc_status
 OK, ERR_VALUE, ERR_RESOURCE, ERR_HIP, ERR_UNSUPPORTED = range(5)

@@ -45,6 +45,29 @@ __device__ __forceinline__ void eig22(F M00, F M01, F M11, int descending, int a
   }
 }
 
+// cos(x) for |x| <= 2.1, in double: the Taylor series of cos to the x^26
+// term (truncation below 1e-18 on that range), by Horner's rule in x^2.  The
+// three angles of the closed form below, (phi + k pi) / 3 with phi in
+// [0, pi], lie in [-pi/3, 2 pi/3], so the general argument reduction of the
+// library cos is not needed.
+__device__ __forceinline__ double cos_small(double x) {
+  const double z = x * x;
+  double p = -1.0 / 403291461126605635584000000.0;  // -1/26!
+  p = fma(p, z, 1.0 / 620448401733239439360000.0);  // 1/24!
+  p = fma(p, z, -1.0 / 1124000727777607680000.0);   // -1/22!
+  p = fma(p, z, 1.0 / 2432902008176640000.0);       // 1/20!
+  p = fma(p, z, -1.0 / 6402373705728000.0);         // -1/18!
+  p = fma(p, z, 1.0 / 20922789888000.0);            // 1/16!
+  p = fma(p, z, -1.0 / 87178291200.0);              // -1/14!
+  p = fma(p, z, 1.0 / 479001600.0);                 // 1/12!
+  p = fma(p, z, -1.0 / 3628800.0);                  // -1/10!
+  p = fma(p, z, 1.0 / 40320.0);                     // 1/8!
+  p = fma(p, z, -1.0 / 720.0);                      // -1/6!
+  p = fma(p, z, 1.0 / 24.0);                        // 1/4!
+  p = fma(p, z, -0.5);                              // -1/2!
+  return fma(p, z, 1.0);
+}
+
 // 3x3, Deledalle et al. (2017) closed form in double (corner.py:566-612) for
 // the matrix [[a, d, f], [d, b, e], [f, e, c]], then the reference's three
 // compare-swap steps (corner.py:613-647).  With abs_sort the comparison keys
@@ -73,9 +96,9 @@ __device__ __forceinline__ void eig33(F aa, F bb, F cc, F dd, F ee, F ff, int de
   }
   const double x1_term = (2.0 / 3.0) * sqrt(x1);
   const double abc = (a + b + c) / 3.0;
-  F lam1 = (F)(abc - x1_term * cos(phi / 3.0));
-  F lam2 = (F)(abc + x1_term * cos((phi - M_PI) / 3.0));
-  F lam3 = (F)(abc + x1_term * cos((phi + M_PI) / 3.0));
+  F lam1 = (F)(abc - x1_term * cos_small(phi / 3.0));
+  F lam2 = (F)(abc + x1_term * cos_small((phi - M_PI) / 3.0));
+  F lam3 = (F)(abc + x1_term * cos_small((phi + M_PI) / 3.0));
   // the network runs on (var1, lam2, var3) = (lam1, lam2, lam3) ascending,
   // (lam3, lam2, lam1) descending
   F& v1 = descending ? lam3 : lam1;

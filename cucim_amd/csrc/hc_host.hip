@@ -137,8 +137,8 @@ int hc_tune_set(int knob, int value) {
   if (knob == HC_TUNE_CONTIG_SEG && value != 0 &&
       (value < 256 || value > 4096 || value % 256 != 0))
     return hc::fail(HC_ERR_VALUE, "segment must be a multiple of 256 in [256, 4096]");
-  if (knob == HC_TUNE_HESSIAN_MFMA && value != 0 && value != 1)
-    return hc::fail(HC_ERR_VALUE, "the Hessian matrix-core switch takes 0 or 1");
+  if (knob == HC_TUNE_HESSIAN_VALU && value != 0 && value != 1)
+    return hc::fail(HC_ERR_VALUE, "the Hessian vector-ALU switch takes 0 or 1");
   if (knob == HC_TUNE_SEP2D_TY && value != 0 && value != 16 && value != 32 && value != 64)
     return hc::fail(HC_ERR_VALUE, "tile rows must be 16, 32 or 64");
   hc::g_tune[knob].store(value);

@@ -271,7 +271,10 @@ constexpr int mfma_npf() {
 template <int NBW, int NS>
 __device__ __forceinline__ void chain_pass_mfma(f32x4_t (&acc)[NS][NBW], const float* X,
                                                 const float* const (&wp)[NS], int nd,
-                                                int nblk, int wv, int lane) {
+                                                int nblk, int wv, int lane, int Km) {
+  // steps of 4 inputs that the band of 16 outputs needs (the last 16-row
+  // block is cut short)
+  const int nsteps = (Km + 15 + 3) / 4;
   const int kk = lane >> 4, ii = lane & 15;
 #pragma unroll
   for (int i = 0; i < NBW; ++i) {
@@ -286,6 +289,7 @@ __device__ __forceinline__ void chain_pass_mfma(f32x4_t (&acc)[NS][NBW], const f
   for (int d = 0; d < nd; ++d) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
+      if (4 * d + u >= nsteps) break;
       const int wi = 15 + 16 * d + 4 * u + kk - ii;
       float av[NS];
 #pragma unroll
@@ -336,7 +340,7 @@ __device__ __forceinline__ void chain_pass(bool exact, f32x4_t (&acc)[NS][NBW], 
   if (exact)
     chain_pass_exact<NBW, NS>(acc, X, wp, Km, nblk, wv, lane);
   else
-    chain_pass_mfma<NBW, NS>(acc, X, wp, nd, nblk, wv, lane);
+    chain_pass_mfma<NBW, NS>(acc, X, wp, nd, nblk, wv, lane, Km);
 }
 
 __device__ __forceinline__ float load_in(const void* in, int dt, int64_t g, bool negate) {
@@ -356,29 +360,38 @@ __device__ __forceinline__ void stage_trimmed(float* wpS, float* wpD, const TapP
   }
 }
 
-// Boundary-extend rows of position outside [0, N) from the rows inside.
-__device__ __forceinline__ void fill_halo(float* X, float* Y, int E, int64_t N, int shift,
-                                          int mode, float cval, int tid) {
+// Source row of every halo row of the line buffer (rows of position outside
+// [0, N)): shift + the boundary-mapped position, or -1 for the constant.
+// The same for every column and buffer, so computed once per work-group.
+__device__ __forceinline__ void halo_table(int* hsrc, int E, int64_t N, int shift, int mode,
+                                           int tid) {
   const int nh = E - (int)N;
-  for (int e = tid; e < nh * kChC; e += 256) {
-    const int h = e / kChC, c = e - h * kChC;
+  for (int h = tid; h < nh; h += 256) {
     const int q = h < shift ? h : h + (int)N;
     const int64_t r = remap((int64_t)q - shift, N, mode);
-    float vx = cval, vy = cval;
-    if (r >= 0) {
-      vx = X[(r + shift) * kChS + c];
-      if (Y) vy = Y[(r + shift) * kChS + c];
-    }
-    X[q * kChS + c] = vx;
-    if (Y) Y[q * kChS + c] = vy;
+    hsrc[h] = r >= 0 ? (int)r + shift : -1;
+  }
+}
+
+// Boundary-extend the halo rows of X (and Y) from the rows inside.
+__device__ __forceinline__ void fill_halo(float* X, float* Y, int E, int N, int shift,
+                                          const int* hsrc, float cval, int tid) {
+  const int nh = E - N;
+  for (int e = tid; e < nh * kChC; e += 256) {
+    const int h = e >> 4, c = e & 15;
+    const int q = h < shift ? h : h + N;
+    const int src = hsrc[h];
+    X[q * kChS + c] = src >= 0 ? X[src * kChS + c] : cval;
+    if (Y) Y[q * kChS + c] = src >= 0 ? Y[src * kChS + c] : cval;
   }
 }
 
 // Stages (tile, job) per work-group of the matrix-core chain pass.
-constexpr int kChainSPW = 2;
+constexpr int kChainSPW = 1;
 
 template <int NBW>
-__global__ __launch_bounds__(256) void k_chain_mfma(const ChainArgs<float> a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_chain_mfma(
+    const ChainArgs<float> a) {
   constexpr int NPF = mfma_npf<NBW>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int E = a.E, Nr = a.Nr, Km = a.Km, shift = a.shift;
@@ -388,9 +401,11 @@ __global__ __launch_bounds__(256) void k_chain_mfma(const ChainArgs<float> a) {
   float* bufB = bufA + E * kChS;
   float* wpS = bufB + E * kChS;
   float* wpD = wpS + WP;
+  int* hsrc = reinterpret_cast<int*>(wpD + WP);
   const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
   const int kk = lane >> 4, ii = lane & 15;
   stage_trimmed(wpS, wpD, a.S, a.D, a.t0, Km, WP, tid);
+  halo_table(hsrc, E, a.N, shift, a.mode, tid);
   const int64_t N = a.N, inner = a.inner, nlines = a.nlines;
   const int mode = a.mode, nblk = Nr / 16;
   const int64_t ntile = (nlines + kChC - 1) / kChC, nst = ntile * a.njobs;
@@ -433,7 +448,7 @@ __global__ __launch_bounds__(256) void k_chain_mfma(const ChainArgs<float> a) {
     const bool skip0 = __syncthreads_or(bad) || cval_bad;
     // the work-group's next stage is in flight during this one's passes
     if (st + 1 < nst && st + 1 < st0 + kChainSPW) load(pf, st + 1);
-    fill_halo(bufA, nullptr, E, N, shift, mode, a.cval, tid);
+    fill_halo(bufA, nullptr, E, (int)N, shift, hsrc, a.cval, tid);
     __syncthreads();
     bool need_s = false, need_d = false;
     for (int k = 0; k < jb.nout; ++k) {
@@ -470,7 +485,7 @@ __global__ __launch_bounds__(256) void k_chain_mfma(const ChainArgs<float> a) {
       }
     }
     const bool skip1 = __syncthreads_or(bad1) || skip0;
-    fill_halo(bufA, bufB, E, N, shift, mode, a.cval, tid);
+    fill_halo(bufA, bufB, E, (int)N, shift, hsrc, a.cval, tid);
     __syncthreads();
     // second passes, per source buffer
     const int64_t L = L0 + ii;
@@ -849,18 +864,23 @@ __device__ __forceinline__ void final_load(const FinalArgs<float>& a, const Fina
   }
 }
 
+// Whether the final pass loads input J + 1 into registers while input J is
+// computed (more registers per lane, fewer resident waves).
+constexpr bool kFinalPrefetch = false;
+
 // Input J of the tile: store the loaded input, start the next load, first
 // op, re-extend, second op into res.
 template <int NIN, int NBW, int J>
 __device__ __forceinline__ void final_input_mfma(const FinalArgs<float>& a, const FinalGeom& g,
                                                  float* xin, float* xf, const float* wpS,
-                                                 const float* wpD, int64_t tile,
+                                                 const float* wpD, const int* hsrc, int64_t tile,
                                                  float (&pf)[mfma_npf<NBW>()],
                                                  f32x4_t (&res)[NBW]) {
   const int E = g.E, shift = g.shift;
   const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
   const int kk = lane >> 4, ii = lane & 15;
   constexpr int NPF = mfma_npf<NBW>();
+  if (!kFinalPrefetch) final_load<NPF>(a, g, pf, J, tile, tid);
   bool bad = false;
 #pragma unroll
   for (int u = 0; u < NPF; ++u) {
@@ -871,8 +891,9 @@ __device__ __forceinline__ void final_input_mfma(const FinalArgs<float>& a, cons
     }
   }
   const bool skip0 = __syncthreads_or(bad) || !finite(a.cval);
-  if (J + 1 < NIN) final_load<NPF>(a, g, pf, J + 1 < NIN ? J + 1 : 0, tile, tid);
-  fill_halo(xin, nullptr, E, g.N, shift, a.mode, a.cval, tid);
+  if (kFinalPrefetch && J + 1 < NIN)
+    final_load<NPF>(a, g, pf, J + 1 < NIN ? J + 1 : 0, tile, tid);
+  fill_halo(xin, nullptr, E, (int)g.N, shift, hsrc, a.cval, tid);
   __syncthreads();
   f32x4_t f[1][NBW];
   {
@@ -890,7 +911,7 @@ __device__ __forceinline__ void final_input_mfma(const FinalArgs<float>& a, cons
     }
   }
   const bool skip1 = __syncthreads_or(bad1) || skip0;
-  fill_halo(xf, nullptr, E, g.N, shift, a.mode, a.cval, tid);
+  fill_halo(xf, nullptr, E, (int)g.N, shift, hsrc, a.cval, tid);
   __syncthreads();
   {
     const float* const wp[1] = {a.second[J] ? wpD : wpS};
@@ -906,15 +927,15 @@ __device__ __forceinline__ void final_input_mfma(const FinalArgs<float>& a, cons
 template <int NIN, int NBW, size_t... Js>
 __device__ __forceinline__ void final_all_mfma(const FinalArgs<float>& a, const FinalGeom& g,
                                                float* xin, float* xf, const float* wpS,
-                                               const float* wpD, int64_t tile,
+                                               const float* wpD, const int* hsrc, int64_t tile,
                                                float (&pf)[mfma_npf<NBW>()],
                                                f32x4_t (&res)[NIN][NBW],
                                                std::index_sequence<Js...>) {
-  (final_input_mfma<NIN, NBW, (int)Js>(a, g, xin, xf, wpS, wpD, tile, pf, res[Js]), ...);
+  (final_input_mfma<NIN, NBW, (int)Js>(a, g, xin, xf, wpS, wpD, hsrc, tile, pf, res[Js]), ...);
 }
 
 template <int NIN, int NBW>
-__global__ __launch_bounds__(256) void k_final_mfma(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_final_mfma(
     const FinalArgs<float> a) {
   constexpr int NPF = mfma_npf<NBW>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -933,27 +954,26 @@ __global__ __launch_bounds__(256) void k_final_mfma(
   float* xf = xin + g.E * kChS;
   float* wpS = xf + g.E * kChS;
   float* wpD = wpS + WP;
+  int* hsrc = reinterpret_cast<int*>(wpD + WP);
   const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
   const int kk = lane >> 4, ii = lane & 15;
   stage_trimmed(wpS, wpD, a.S, a.D, a.t0, a.Km, WP, tid);
+  halo_table(hsrc, g.E, g.N, g.shift, a.mode, tid);
   double gsq = a.gamma_sq;
   if (a.epi == EPI_FRANGI && a.gamma_sq_dev) gsq = *a.gamma_sq_dev;
   const int64_t N = g.N;
   float rgmax = 0.f;
   float pf[NPF];
   const int64_t tile = xcd_swizzle(blockIdx.x, g.G);
-  final_load<NPF>(a, g, pf, 0, tile, tid);
+  if (kFinalPrefetch) final_load<NPF>(a, g, pf, 0, tile, tid);
   {
     f32x4_t res[NIN][NBW];
-    final_all_mfma<NIN, NBW>(a, g, xin, xf, wpS, wpD, tile, pf, res,
+    final_all_mfma<NIN, NBW>(a, g, xin, xf, wpS, wpD, hsrc, tile, pf, res,
                              std::make_index_sequence<NIN>{});
     // epilogue (k_final_contig's)
     const int64_t row = tile * kChC + ii;
     if (row < g.nrows) {
-      // one voxel per iteration (a rolled loop: the eigen solves of all
-      // the lane's voxels would not fit in registers beside the tile's);
-      // the voxel's elements are picked from the registers by selects
-#pragma unroll 1
+#pragma unroll
       for (int v = 0; v < 4 * NBW; ++v) {
         const int i = v >> 2;
         if (wv + 4 * i >= g.nblk) break;
@@ -961,12 +981,7 @@ __global__ __launch_bounds__(256) void k_final_mfma(
         if (p >= N) continue;
         float h[NIN];
 #pragma unroll
-        for (int j = 0; j < NIN; ++j) {
-          float x = res[j][0][0];
-#pragma unroll
-          for (int w = 1; w < 4 * NBW; ++w) x = v == w ? res[j][w >> 2][w & 3] : x;
-          h[j] = x;
-        }
+        for (int j = 0; j < NIN; ++j) h[j] = res[j][i][v & 3];
         const int64_t idx = row * N + p;
         if (a.epi == EPI_STORE) {
 #pragma unroll
@@ -1152,9 +1167,8 @@ int axis_taps(const hc_taps* ts, const hc_taps* td, AxisTaps<W>* out) {
   return HC_OK;
 }
 
-// Matrix-core passes: off unless the HC_TUNE_HESSIAN_MFMA knob is set (they
-// measure slower than the vector-ALU passes on the 256^3 workload).
-inline bool chain_mfma_enabled() { return tune_get(HC_TUNE_HESSIAN_MFMA) == 1; }
+// Matrix-core passes: on unless the HC_TUNE_HESSIAN_VALU knob is set.
+inline bool chain_mfma_enabled() { return tune_get(HC_TUNE_HESSIAN_VALU) == 0; }
 
 // Geometry of the matrix-core passes over lines of N: the nonzero span of
 // both tap sets (zero taps at the ends add exact zeros for finite inputs;
@@ -1182,7 +1196,7 @@ inline bool mfma_geom(const TapPack<float>& S, const TapPack<float>& D, int64_t 
   // the last input block reads row Nr + 16 nd - 17 <= Nr + Km + 13
   g.E = g.Nr + g.Km + 14;
   g.nd = (g.Km + 30) / 16;
-  g.lds = (2 * g.E * kChS + 2 * (16 * g.nd + 16)) * (int)sizeof(float);
+  g.lds = (2 * g.E * kChS + 2 * (16 * g.nd + 16) + g.E) * (int)sizeof(float);
   return g.lds <= kMaxLds && N <= 16 * npf;
 }
 

@@ -107,9 +107,10 @@ typedef enum hc_tune_knob {
                                segment, 256..4096, multiple of 256 */
   HC_TUNE_SEP2D_TY = 1,     /* fused 2-D separable: output rows per tile,
                                16, 32 or 64 */
-  HC_TUNE_HESSIAN_MFMA = 2, /* 1: float32 fused Hessian passes on the
-                               matrix cores where they apply (default 0:
-                               vector ALUs, the reference's arithmetic) */
+  HC_TUNE_HESSIAN_VALU = 2, /* 1: fused Hessian passes on the vector ALUs
+                               only, tap by tap as the reference computes
+                               (default 0: float32 passes on the matrix
+                               cores where they apply) */
   HC_TUNE_NKNOBS = 8
 } hc_tune_knob;
 int hc_tune_set(int knob, int value);

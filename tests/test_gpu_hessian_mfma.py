@@ -2,9 +2,9 @@
 
 The float32 passes of the fused Hessian (strided line passes for lines of up
 to 256 voxels, the final pass along the contiguous axis for rows of up to 256
-voxels in 3-D and 512 in 2-D) run on the matrix cores when the
-HC_TUNE_HESSIAN_MFMA knob is set; by default they run on the vector ALUs,
-tap by tap as the reference computes them.  Tolerances (float32 data):
+voxels in 3-D and 512 in 2-D) run on the matrix cores by default; the
+HC_TUNE_HESSIAN_VALU knob switches them to the vector-ALU passes, tap by tap
+as the reference computes them.  Tolerances (float32 data):
 Hessian elements within 2e-6 * max|H| of the FP64 oracle and of the VALU
 passes; Frangi within the plain Frangi's bounds.  A work-group of lines that
 holds a non-finite value is computed tap by tap, so the non-finite pattern of
@@ -30,14 +30,14 @@ def _np(t):
     return t.cpu().numpy()
 
 
-class _Mfma:
-    """Matrix-core passes on inside the block."""
+class _Valu:
+    """Vector-ALU passes only inside the block."""
 
     def __enter__(self):
-        _lib.check(_lib.lib().hc_tune_set(_lib.TUNE_HESSIAN_MFMA, 1))
+        _lib.check(_lib.lib().hc_tune_set(_lib.TUNE_HESSIAN_VALU, 1))
 
     def __exit__(self, *exc):
-        _lib.lib().hc_tune_set(_lib.TUNE_HESSIAN_MFMA, 0)
+        _lib.lib().hc_tune_set(_lib.TUNE_HESSIAN_VALU, 0)
 
 
 def _hessian(x, sigma, mode, valu):
@@ -47,9 +47,9 @@ def _hessian(x, sigma, mode, valu):
         return [_np(h) for h in hessian_matrix(_t(x), sigma, mode=mode, cval=0.0,
                                                use_gaussian_derivatives=True)]
 
-    if valu:
+    if not valu:
         return run()
-    with _Mfma():
+    with _Valu():
         return run()
 
 
@@ -101,11 +101,10 @@ def test_mfma_frangi_region_alignment(gpu):
     sig = [1.0, 2.0]
     reg_in = x[16:96, 32:112].contiguous()
     full, reg = torch.empty_like(x), torch.empty_like(reg_in)
-    with _Mfma():
-        plan = ridges.FrangiPlan(tuple(x.shape), np.float32, sig)
-        assert plan.run(x, full, gamma=0.5, black_ridges=False) == 0
-        plan2 = ridges.FrangiPlan(tuple(reg_in.shape), np.float32, sig)
-        assert plan2.run(reg_in, reg, gamma=0.5, black_ridges=False) == 0
+    plan = ridges.FrangiPlan(tuple(x.shape), np.float32, sig)
+    assert plan.run(x, full, gamma=0.5, black_ridges=False) == 0
+    plan2 = ridges.FrangiPlan(tuple(reg_in.shape), np.float32, sig)
+    assert plan2.run(reg_in, reg, gamma=0.5, black_ridges=False) == 0
     # reach of the two passes per axis: sigma 2 -> 2 * int(8 * sqrt(2) + 0.5)
     # = 22 voxels; sigma 1's float32 taps are zero beyond 10 per pass
     reach = 24
@@ -116,5 +115,5 @@ def test_mfma_frangi_region_alignment(gpu):
 
 def test_hessian_knob_values(gpu):
     lib = _lib.lib()
-    assert lib.hc_tune_set(_lib.TUNE_HESSIAN_MFMA, 2) == _lib.ERR_VALUE
-    assert lib.hc_tune_get(_lib.TUNE_HESSIAN_MFMA) == 0
+    assert lib.hc_tune_set(_lib.TUNE_HESSIAN_VALU, 2) == _lib.ERR_VALUE
+    assert lib.hc_tune_get(_lib.TUNE_HESSIAN_VALU) == 0
