@@ -155,3 +155,11 @@ def get_output(output, inp, shape=None):
     if tuple(out.shape) != shape:
         raise RuntimeError("output shape not correct")
     return out
+
+
+def binary(a, b, out, op):
+    """out = a - b (``_lib.BIN_SUB``) or a * b (``_lib.BIN_MUL``), one rounding
+    per element in the arrays' float dtype; out may be a or b."""
+    _lib.check(_lib.lib().hc_binary(hc_array(a), hc_array(b), hc_array(out), op, stream()),
+               "elementwise op")
+    return out

@@ -27,6 +27,7 @@ ALGO_AUTO, ALGO_TILED, ALGO_GATHER = range(3)
 
 # hc_tune_knob
 TUNE_CONTIG_SEG, TUNE_SEP2D_TY, TUNE_HESSIAN_VALU = 0, 1, 2
+BIN_SUB, BIN_MUL = 0, 1
 
 # hc_status
 OK, ERR_VALUE, ERR_RESOURCE, ERR_HIP, ERR_UNSUPPORTED = range(5)
@@ -54,6 +55,7 @@ ENTRY_POINTS = (
     "hc_mask_eroded",
     "hc_maximum_inplace",
     "hc_square_acc",
+    "hc_binary",
     "hc_sqrt_inplace",
     "hc_copy_cast",
     "hc_batch2d_plan",
@@ -130,6 +132,7 @@ _SIGNATURES = {
     "hc_mask_eroded": ([_A, _A, _v], _i),
     "hc_maximum_inplace": ([_A, _A, _v], _i),
     "hc_square_acc": ([_A, _A, _i, _v], _i),
+    "hc_binary": ([_A, _A, _A, _i, _v], _i),
     "hc_sqrt_inplace": ([_A, _v], _i),
     "hc_copy_cast": ([_A, _A, _v], _i),
     "hc_batch2d_plan": ([_v, _i, _i, _i, _v, ctypes.c_int64], ctypes.c_int64),

@@ -236,6 +236,16 @@ __global__ __launch_bounds__(256) void k_square_acc(F* __restrict__ acc, const F
 }
 
 template <typename F>
+__global__ __launch_bounds__(256) void k_binary(const F* a, const F* b, F* out, int64_t n,
+                                                int op) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    const F x = a[i], y = b[i];
+    out[i] = op == HC_BIN_MUL ? mul_rn(x, y) : add_rn(x, -y);
+  }
+}
+
+template <typename F>
 __global__ __launch_bounds__(256) void k_sqrt(F* __restrict__ a, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * 256)
@@ -510,6 +520,27 @@ int hc_square_acc(const hc_array* acc, const hc_array* v, int first, void* strea
     HC_LAUNCH("k_square_acc", st, k_square_acc<float>, dim3((unsigned)grid_for(n)), dim3(256), 0, (float*)acc->data, (const float*)v->data, n, first);
   else
     return fail(HC_ERR_VALUE, "square_acc needs float32/float64");
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+int hc_binary(const hc_array* a, const hc_array* b, const hc_array* out, int op, void* stream) {
+  if (!a || !b || !out) return fail(HC_ERR_VALUE, "null array");
+  if (op != HC_BIN_SUB && op != HC_BIN_MUL) return fail(HC_ERR_VALUE, "unknown binary op");
+  if (!same_shape(a, b) || !same_shape(a, out) || !is_c_contiguous(a) || !is_c_contiguous(b) ||
+      !is_c_contiguous(out) || a->dtype != b->dtype || a->dtype != out->dtype)
+    return fail(HC_ERR_VALUE, "arrays must be C-contiguous with equal shapes and dtypes");
+  const int64_t n = numel(a);
+  if (n == 0) return HC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (a->dtype == HC_FLOAT64)
+    HC_LAUNCH("k_binary", st, k_binary<double>, dim3((unsigned)grid_for(n)), dim3(256), 0,
+              (const double*)a->data, (const double*)b->data, (double*)out->data, n, op);
+  else if (a->dtype == HC_FLOAT32)
+    HC_LAUNCH("k_binary", st, k_binary<float>, dim3((unsigned)grid_for(n)), dim3(256), 0,
+              (const float*)a->data, (const float*)b->data, (float*)out->data, n, op);
+  else
+    return fail(HC_ERR_VALUE, "binary ops need float32/float64");
   HC_HIP(hipGetLastError());
   return HC_OK;
 }

@@ -1,1 +1,6 @@
-from .corner import hessian_matrix, hessian_matrix_eigvals  # noqa: F401
+from .corner import (  # noqa: F401
+    hessian_matrix,
+    hessian_matrix_eigvals,
+    structure_tensor,
+    structure_tensor_eigenvalues,
+)

@@ -13,7 +13,8 @@ from .._vendored import ndimage as ndi
 from .._vendored._ndimage_filters import _gaussian_kernel1d, _trimmed
 from .._vendored._ndimage_util import _mode_code
 
-__all__ = ["hessian_matrix", "hessian_matrix_eigvals", "hessian_kernels"]
+__all__ = ["hessian_matrix", "hessian_matrix_eigvals", "hessian_kernels", "structure_tensor",
+           "structure_tensor_eigenvalues"]
 
 
 def hessian_kernels(sigma, ndim, dtype):
@@ -172,3 +173,55 @@ def _symmetric_compute_eigenvalues(S_elems, sort="descending", abs_sort=False):
 def hessian_matrix_eigvals(H_elems):
     """Eigenvalues of the Hessian, decreasing (``corner.py:824-856``)."""
     return _symmetric_compute_eigenvalues(H_elems)
+
+
+def _prepare_grayscale_input_nD(image):
+    """``feature/util.py:19-24``: squeeze, 2-5 dimensions, img_as_float in
+    the supported float type."""
+    from .._shared.utils import check_nD
+    from ..util.dtype import img_as_float
+
+    image = _array.as_tensor(image).squeeze()
+    check_nD(image, range(2, 6))
+    image = img_as_float(image)
+    return astype(image, _supported_float_type(_array.np_dtype(image.dtype)))
+
+
+def _compute_derivatives(image, mode="constant", cval=0):
+    """``corner.py:27-52``: ``ndi.sobel`` along every axis."""
+    return [ndi.sobel(image, axis=i, mode=mode, cval=cval) for i in range(image.dim())]
+
+
+def structure_tensor(image, sigma=1, mode="constant", cval=0, order="rc"):
+    """Structure tensor (``corner.py:55-158``): the Gaussian-smoothed
+    products of the Sobel derivatives, in ``combinations_with_replacement``
+    order of the axes (reversed for ``order='xy'``, 2-D only).  A 3-D image
+    whose last axis has 3 entries is smoothed per channel, as the reference
+    does."""
+    from .._shared.filters import gaussian
+
+    t = _array.as_tensor(image)
+    if order == "xy" and t.dim() > 2:
+        raise ValueError('Only "rc" order is supported for dim > 2.')
+    if order not in ["rc", "xy"]:
+        raise ValueError(f'order {order} is invalid. Must be either "rc" or "xy"')
+    if not np.isscalar(sigma):
+        sigma = tuple(sigma)
+        if len(sigma) != t.dim():
+            raise ValueError("sigma must have as many elements as image has axes")
+    image = _prepare_grayscale_input_nD(t).contiguous()
+    derivatives = _compute_derivatives(image, mode=mode, cval=cval)
+    if order == "xy":
+        derivatives = list(reversed(derivatives))
+    channel_axis = -1 if (image.dim() == 3 and image.shape[-1] == 3) else None
+    out = []
+    for d0, d1 in combinations_with_replacement(derivatives, 2):
+        prod = _array.binary(d0, d1, _array.empty(tuple(d0.shape), _array.np_dtype(d0.dtype)),
+                             _lib.BIN_MUL)
+        out.append(gaussian(prod, sigma=sigma, mode=mode, cval=cval, channel_axis=channel_axis))
+    return out
+
+
+def structure_tensor_eigenvalues(A_elems):
+    """Eigenvalues of the structure tensor, decreasing (``corner.py:786-821``)."""
+    return _symmetric_compute_eigenvalues(A_elems)

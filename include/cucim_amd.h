@@ -315,6 +315,14 @@ int hc_square_acc(const hc_array* acc, const hc_array* v, int first, void* strea
 /* a = sqrt(a), float32/float64, C-contiguous. */
 int hc_sqrt_inplace(const hc_array* a, void* stream);
 
+/* Elementwise out = a - b (HC_BIN_SUB) or a * b (HC_BIN_MUL), each result
+ * rounded once in the arrays' dtype (float32/float64; equal shapes, dtypes,
+ * C-contiguous; out may alias a or b).  The `out -= gaussian(...)` of
+ * filters/_gaussian.py:154-172 and the `der0 * der1` products of
+ * feature/corner.py:137-148. */
+enum { HC_BIN_SUB = 0, HC_BIN_MUL = 1 };
+int hc_binary(const hc_array* a, const hc_array* b, const hc_array* out, int op, void* stream);
+
 /* Strided copy with dtype cast (cast<> semantics of
  * _ndimage_filters_core.py:227-244): out[...] = cast(in[...]). */
 int hc_copy_cast(const hc_array* in, const hc_array* out, void* stream);

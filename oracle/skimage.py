@@ -19,6 +19,10 @@ Reference files (relative to ``python/cucim/src/cucim/skimage/``):
   filters/ridges.py:50-225         meijering, sato
   filters/ridges.py:479-565        hessian (hybrid Hessian filter)
   _shared/_gradient.py             finite-difference gradient
+  filters/_gaussian.py:13-174      difference_of_gaussians
+  feature/corner.py:27-158         structure_tensor (_compute_derivatives)
+  feature/util.py:19-24            _prepare_grayscale_input_nD
+  feature/corner.py:786-821        structure_tensor_eigenvalues
 """
 
 import math
@@ -558,3 +562,72 @@ def hessian(image, sigmas=range(1, 10, 2), scale_range=None, scale_step=None, al
                  cval=cval, ndi=ndi, dtype=dtype)
     out[out <= 0] = 1
     return out
+
+
+def difference_of_gaussians(image, low_sigma, high_sigma=None, *, mode="nearest", cval=0,
+                            channel_axis=None, truncate=4.0, ndi=None, dtype=None):
+    """``filters/_gaussian.py:13-174``: img_as_float, sigma normalisation and
+    checks, gaussian(low) - gaussian(high) with preserve_range=False."""
+    image = img_as_float(image)
+    spatial_dims = image.ndim - 1 if channel_axis is not None else image.ndim
+    if np.isscalar(low_sigma):
+        low_sigma = (low_sigma,) * spatial_dims
+    low_sigma = tuple(map(float, low_sigma))
+    if high_sigma is None:
+        high_sigma = tuple(low * 1.6 for low in low_sigma)
+    elif np.isscalar(high_sigma):
+        high_sigma = (high_sigma,) * spatial_dims
+    high_sigma = tuple(map(float, high_sigma))
+    if len(low_sigma) != 1 and len(low_sigma) != spatial_dims:
+        raise ValueError("low_sigma must have length equal to number of spatial dimensions")
+    if len(high_sigma) != 1 and len(high_sigma) != spatial_dims:
+        raise ValueError("high_sigma must have length equal to number of spatial dimensions")
+    if any(s_hi < s_low for s_hi, s_low in zip(high_sigma, low_sigma)):
+        raise ValueError("high_sigma must be equal to or larger than low_sigma for all axes")
+    kw = dict(mode=mode, cval=cval, channel_axis=channel_axis, truncate=truncate,
+              preserve_range=False, ndi=ndi, dtype=dtype)
+    out = gaussian(image, sigma=low_sigma, **kw)
+    return out - gaussian(image, sigma=high_sigma, **kw)
+
+
+def _prepare_grayscale_input_nD(image):
+    """``feature/util.py:19-24``: squeeze, 2-5 dimensions, img_as_float."""
+    image = np.squeeze(np.asarray(image))
+    if image.ndim not in range(2, 6):
+        raise ValueError("The parameter `image` must be a 2-5-dimensional array")
+    return img_as_float(image)
+
+
+def structure_tensor(image, sigma=1, mode="constant", cval=0, order="rc", *, ndi=None,
+                     dtype=None):
+    """``feature/corner.py:55-158``: Sobel derivatives along every axis
+    (``_compute_derivatives``, ``:27-52``), their pairwise products in
+    ``combinations_with_replacement`` order, each Gaussian-smoothed with the
+    skimage gaussian (a 3-D image whose last axis has 3 entries is smoothed
+    per channel)."""
+    ndi_mod = ondi if ndi is None else ndi
+    image = np.asarray(image)
+    if order == "xy" and image.ndim > 2:
+        raise ValueError('Only "rc" order is supported for dim > 2.')
+    if order not in ["rc", "xy"]:
+        raise ValueError(f'order {order} is invalid. Must be either "rc" or "xy"')
+    if not np.isscalar(sigma):
+        sigma = tuple(sigma)
+        if len(sigma) != image.ndim:
+            raise ValueError("sigma must have as many elements as image has axes")
+    image = _prepare_grayscale_input_nD(image)
+    if dtype is not None:
+        image = image.astype(dtype)
+    derivatives = [ndi_mod.sobel(image, axis=i, mode=mode, cval=cval)
+                   for i in range(image.ndim)]
+    if order == "xy":
+        derivatives = list(reversed(derivatives))
+    channel_axis = -1 if (image.ndim == 3 and image.shape[-1] == 3) else None
+    return [gaussian(d0 * d1, sigma=sigma, mode=mode, cval=cval, channel_axis=channel_axis,
+                     ndi=ndi, dtype=dtype)
+            for d0, d1 in combinations_with_replacement(derivatives, 2)]
+
+
+def structure_tensor_eigenvalues(A_elems):
+    """``feature/corner.py:786-821``: descending, by value."""
+    return symmetric_compute_eigenvalues(A_elems)
