@@ -115,6 +115,7 @@ struct TiledArgs {
 constexpr int kStrCols = 64;     // strided: lines per work-group
 constexpr int kStrStride = 65;   // strided: LDS row stride (elements)
 constexpr int kStrTA = 64;       // strided: outputs per line per work-group
+constexpr int kStrWaveBlocks = kStrTA / 64;  // 16-output blocks per wave
 
 // 16-byte loads each lane keeps in flight while staging.
 constexpr int kVecBatch = 8;
@@ -231,14 +232,15 @@ __global__ __launch_bounds__(256) void k_tiled_contig(const TiledArgs<W> a) {
       conv_row4<SK>(acc, ext + o0, wl, a.tp);
       if (VEC && o + 4 <= a.N) {
         W* op = reinterpret_cast<W*>(a.out) + lbase + o;
+        // streaming stores: the output is not read again by this launch
         if (sizeof(W) == 4) {
           typename Vec16<float>::T t = {(float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]};
-          *reinterpret_cast<typename Vec16<float>::T*>(op) = t;
+          __builtin_nontemporal_store(t, reinterpret_cast<typename Vec16<float>::T*>(op));
         } else {
           typename Vec16<double>::T t0 = {(double)acc[0], (double)acc[1]};
           typename Vec16<double>::T t1 = {(double)acc[2], (double)acc[3]};
-          reinterpret_cast<typename Vec16<double>::T*>(op)[0] = t0;
-          reinterpret_cast<typename Vec16<double>::T*>(op)[1] = t1;
+          __builtin_nontemporal_store(t0, reinterpret_cast<typename Vec16<double>::T*>(op));
+          __builtin_nontemporal_store(t1, reinterpret_cast<typename Vec16<double>::T*>(op) + 1);
         }
       } else {
 #pragma unroll
@@ -279,21 +281,28 @@ __global__ __launch_bounds__(256) void k_tiled_strided(const TiledArgs<W> a) {
       const W* in = reinterpret_cast<const W*>(a.in);
       const int64_t Lt = lb * kStrCols;
       const int total = R * CPR;
+      // 256 is a multiple of CPR: the thread's chunk column is fixed, so its
+      // line base is computed once
+      static_assert(256 % CPR == 0, "chunk column per thread");
+      const int cc = (threadIdx.x % CPR) * VN;
+      const int64_t Lc = Lt + cc;
+      const bool cok = Lc < a.nlines;
+      const int64_t lo2 = cok ? Lc / inner : 0;
+      const W* colp = in + lo2 * N * inner + (Lc - lo2 * inner);
       bad = false;
       for (int e0 = threadIdx.x; e0 < total; e0 += 256 * kVecBatch) {
         W v[kVecBatch][VN];
 #pragma unroll
         for (int u = 0; u < kVecBatch; ++u) {
           const int e = e0 + 256 * u;
-          const int q = e / CPR, c = (e - q * CPR) * VN;
+          const int q = e / CPR;
 #pragma unroll
           for (int j = 0; j < VN; ++j) v[u][j] = a.cval;
-          const int64_t Lc = Lt + c;
-          if (e < total && Lc < a.nlines) {
-            const int64_t r = remap(p0 + q, N, mode);
+          if (e < total && cok) {
+            const int64_t pq = p0 + q;
+            const int64_t r = (pq >= 0 && pq < N) ? pq : remap(pq, N, mode);
             if (r >= 0) {
-              const int64_t lo2 = Lc / inner, li2 = Lc - lo2 * inner;
-              const V t = *reinterpret_cast<const V*>(in + lo2 * N * inner + li2 + r * inner);
+              const V t = *reinterpret_cast<const V*>(colp + r * inner);
 #pragma unroll
               for (int j = 0; j < VN; ++j) v[u][j] = t[j];
             }
@@ -303,10 +312,10 @@ __global__ __launch_bounds__(256) void k_tiled_strided(const TiledArgs<W> a) {
         for (int u = 0; u < kVecBatch; ++u) {
           const int e = e0 + 256 * u;
           if (e < total) {
-            const int q = e / CPR, c = (e - q * CPR) * VN;
+            const int q = e / CPR;
 #pragma unroll
             for (int j = 0; j < VN; ++j) {
-              ext[q * kStrStride + c + j] = v[u][j];
+              ext[q * kStrStride + cc + j] = v[u][j];
               bad |= !finite(v[u][j]);
             }
           }
@@ -325,25 +334,29 @@ __global__ __launch_bounds__(256) void k_tiled_strided(const TiledArgs<W> a) {
   }
   const bool skip = __syncthreads_or(bad);
   if (!lok) return;
-  const int64_t n0 = a0 + 16 * wv;
-  if (n0 >= a.N) return;
-  with_skip(skip, [&](auto sk) {
-    constexpr bool SK = decltype(sk)::value;
-    W acc[16];
+#pragma unroll 1
+  for (int blk = 0; blk < kStrWaveBlocks; ++blk) {
+    const int rb = 16 * (wv + 4 * blk);  // first output row of the block
+    const int64_t n0 = a0 + rb;
+    if (n0 >= a.N) return;
+    with_skip(skip, [&](auto sk) {
+      constexpr bool SK = decltype(sk)::value;
+      W acc[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) acc[j] = (W)0;
-    conv_col16<SK, W, kStrStride>(acc, ext + (16 * wv) * kStrStride + lane, wl, a.tp);
-    if (VEC) {
-      W* op = reinterpret_cast<W*>(a.out) + lbase;
+      for (int j = 0; j < 16; ++j) acc[j] = (W)0;
+      conv_col16<SK, W, kStrStride>(acc, ext + rb * kStrStride + lane, wl, a.tp);
+      if (VEC) {
+        W* op = reinterpret_cast<W*>(a.out) + lbase;
 #pragma unroll
-      for (int j = 0; j < 16; ++j)
-        if (n0 + j < a.N) op[(n0 + j) * a.inner] = acc[j];
-    } else {
+        for (int j = 0; j < 16; ++j)
+          if (n0 + j < a.N) __builtin_nontemporal_store(acc[j], op + (n0 + j) * a.inner);
+      } else {
 #pragma unroll
-      for (int j = 0; j < 16; ++j)
-        if (n0 + j < a.N) store_cast<W>(a.out, lbase + (n0 + j) * a.inner, a.out_dt, acc[j]);
-    }
-  });
+        for (int j = 0; j < 16; ++j)
+          if (n0 + j < a.N) store_cast<W>(a.out, lbase + (n0 + j) * a.inner, a.out_dt, acc[j]);
+      }
+    });
+  }
 }
 
 // ------------------------------------------------------ fused 2-D --
@@ -619,6 +632,13 @@ int launch_tiled_1d(const hc_array* in, const hc_array* out, const hc_taps* taps
                      (ln.inner * (int64_t)sizeof(W)) % 16 == 0 &&
                      (((uintptr_t)in->data | (uintptr_t)out->data) % 16) == 0 &&
                      is_c_contiguous(in) && is_c_contiguous(out);
+    if (smem > 160 * 1024) return fail(HC_ERR_RESOURCE, "line tile does not fit in LDS");
+    if (smem > 64 * 1024) {
+      const hipError_t e = hipFuncSetAttribute(
+          vec ? (const void*)k_tiled_strided<W, true> : (const void*)k_tiled_strided<W, false>,
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+      if (e != hipSuccess) return hip_check(e, "hipFuncSetAttribute(k_tiled_strided)");
+    }
     if (vec)
       HC_LAUNCH("k_tiled_strided", st, (k_tiled_strided<W, true>), dim3((unsigned)blocks),
                 dim3(256), smem, a);

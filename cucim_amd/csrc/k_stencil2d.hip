@@ -36,10 +36,22 @@ __global__ __launch_bounds__(256) void k_sep2d_reg(const Reg2dArgs a, const RegT
   const int64_t xr = remap(x0 - a.a1 + c, a.W, a.mode1);
   const int64_t ya = y0 - a.a0;
   float v[TY + K - 1];
+  if (ya >= 0 && ya + TY + K - 1 <= a.H) {
+    // every staged row inside the image (the tile is not at the top or
+    // bottom border): consecutive rows, no boundary mapping
+    const float* col = a.in + ya * a.W + (xr >= 0 ? xr : 0);
 #pragma unroll
-  for (int j = 0; j < TY + K - 1; ++j) {
-    const int64_t yr = remap(ya + j, a.H, a.mode0);
-    v[j] = (yr >= 0 && xr >= 0) ? a.in[yr * a.W + xr] : a.cval;
+    for (int j = 0; j < TY + K - 1; ++j) v[j] = col[(int64_t)j * a.W];
+    if (xr < 0) {
+#pragma unroll
+      for (int j = 0; j < TY + K - 1; ++j) v[j] = a.cval;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < TY + K - 1; ++j) {
+      const int64_t yr = remap(ya + j, a.H, a.mode0);
+      v[j] = (yr >= 0 && xr >= 0) ? a.in[yr * a.W + xr] : a.cval;
+    }
   }
   float acc[TY];
   reg_vpass<K, TY>(acc, v, t0);
@@ -55,8 +67,10 @@ __global__ __launch_bounds__(256) void k_sep2d_reg(const Reg2dArgs a, const RegT
     reg_hpass<K>(r, U + i * kRegUStride + o, t1);
     float* op = a.out + y * a.W + x;
     if (x + kRegP <= a.W && ((uintptr_t)op & 15) == 0) {
-      reinterpret_cast<float4*>(op)[0] = make_float4(r[0], r[1], r[2], r[3]);
-      reinterpret_cast<float4*>(op)[1] = make_float4(r[4], r[5], r[6], r[7]);
+      // streaming stores: the output is not read again by this launch
+      typedef float f4v __attribute__((ext_vector_type(4)));
+      __builtin_nontemporal_store(f4v{r[0], r[1], r[2], r[3]}, reinterpret_cast<f4v*>(op));
+      __builtin_nontemporal_store(f4v{r[4], r[5], r[6], r[7]}, reinterpret_cast<f4v*>(op) + 1);
     } else {
 #pragma unroll
       for (int j = 0; j < kRegP; ++j)
