@@ -214,10 +214,22 @@ __global__ __launch_bounds__(256) void k_tiles_ggs(const GgsArgs a, const RegTap
       const int64_t xr = remap(x0 - 1 - a.ag + c, W, a.gmode);
       const int64_t ya = y0 - 1 - a.ag;
       float v[GR + K - 1];
+      if (ya >= 0 && ya + GR + K - 1 <= H) {
+        // every staged row inside the tile: consecutive rows, no boundary
+        // mapping
+        const uint8_t* col = in + 3 * (ya * W + (xr >= 0 ? xr : 0));
 #pragma unroll
-      for (int j = 0; j < GR + K - 1; ++j) {
-        const int64_t yr = remap(ya + j, H, a.gmode);
-        v[j] = (yr >= 0 && xr >= 0) ? gray_u8(in + 3 * (yr * W + xr), a.scale) : a.gcval;
+        for (int j = 0; j < GR + K - 1; ++j) v[j] = gray_u8(col + 3 * W * j, a.scale);
+        if (xr < 0) {
+#pragma unroll
+          for (int j = 0; j < GR + K - 1; ++j) v[j] = a.gcval;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < GR + K - 1; ++j) {
+          const int64_t yr = remap(ya + j, H, a.gmode);
+          v[j] = (yr >= 0 && xr >= 0) ? gray_u8(in + 3 * (yr * W + xr), a.scale) : a.gcval;
+        }
       }
       float acc[GR];
       reg_vpass<K, GR>(acc, v, tg);
@@ -269,31 +281,54 @@ __global__ __launch_bounds__(256) void k_tiles_ggs(const GgsArgs a, const RegTap
     // 4. edge magnitude, one output column per lane
     if (c < a.TX && x0 + c < W) {
       float* out = reinterpret_cast<float*>(d.out);
+      // a 3x3 window slides down the lane's column: G rows i, i + 1, i + 2
+      // (image rows y - 1 .. y + 1) at columns x - 1 .. x + 1, and the
+      // horizontal edge response of each row, are carried to the next row
+      const float* gc = G + c;
+      auto edge_x = [&](const float (&g)[3]) {
+        float u = 0.f;
+        u = etap(u, g[0], a.e[0]);
+        u = etap(u, g[1], a.e[1]);
+        return etap(u, g[2], a.e[2]);  // edge along axis 1 of one row
+      };
+      float gm[3], g0[3], gp[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        gm[q] = gc[q];
+        g0[q] = gc[kRegUStride + q];
+      }
+      float exm = edge_x(gm), ex0 = edge_x(g0);
+#pragma unroll 4
       for (int i = 0; i < TY; ++i) {
         const int64_t y = y0 + i;
         if (y >= H) break;
-        const float* g0 = G + i * kRegUStride + c;  // rows y-1..y+1, cols x-1..x+1
-        float ey[3], ex[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) gp[q] = gc[(i + 2) * kRegUStride + q];
+        const float exp_ = edge_x(gp);
+        float ey[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
           float v = 0.f;
-          v = etap(v, g0[q], a.e[0]);
-          v = etap(v, g0[kRegUStride + q], a.e[1]);
-          v = etap(v, g0[2 * kRegUStride + q], a.e[2]);
-          ey[q] = v;  // edge along axis 0 at column x - 1 + q
-          float u = 0.f;
-          u = etap(u, g0[q * kRegUStride], a.e[0]);
-          u = etap(u, g0[q * kRegUStride + 1], a.e[1]);
-          u = etap(u, g0[q * kRegUStride + 2], a.e[2]);
-          ex[q] = u;  // edge along axis 1 at row y - 1 + q
+          v = etap(v, gm[q], a.e[0]);
+          v = etap(v, g0[q], a.e[1]);
+          ey[q] = etap(v, gp[q], a.e[2]);  // edge along axis 0 at column x - 1 + q
         }
         float A = 0.f, B = 0.f;
+        A = etap(A, ey[0], a.s[0]);
+        B = etap(B, exm, a.s[0]);
+        A = etap(A, ey[1], a.s[1]);
+        B = etap(B, ex0, a.s[1]);
+        A = etap(A, ey[2], a.s[2]);
+        B = etap(B, exp_, a.s[2]);
+        __builtin_nontemporal_store(sqrtf(add_rn(mul_rn(A, A), mul_rn(B, B))),
+                                    out + y * W + x0 + c);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-          A = etap(A, ey[q], a.s[q]);
-          B = etap(B, ex[q], a.s[q]);
+          gm[q] = g0[q];
+          g0[q] = gp[q];
         }
-        out[y * W + x0 + c] = sqrtf(add_rn(mul_rn(A, A), mul_rn(B, B)));
+        exm = ex0;
+        ex0 = exp_;
       }
     }
     __syncthreads();  // V and G are reused by the next item
