@@ -258,17 +258,14 @@ __device__ __forceinline__ int wave_id() {
   return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 }
 
-// Values per thread of one tile's interior (16 lines of N <= 16 NPF) held
-// in flight.
-template <int NBW>
-constexpr int mfma_npf() {
-  return NBW <= 4 ? 16 : 32;
-}
+// Values per thread of one tile's interior (16 lines of N <= 64 NW, NW
+// waves per work-group) held in flight.
+constexpr int kMfmaNpf = 16;
 
-// Blocks b = wave + 4 i (i < NBW) of 16 positions: acc[s][i] for the NS
+// Blocks b = wave + NW i (i < NBW) of 16 positions: acc[s][i] for the NS
 // weight sets wp[s] (zero-padded: trimmed tap t at wp[15 + t]).  The A
 // fragment of step (d, u) serves every block of the wave.
-template <int NBW, int NS>
+template <int NBW, int NS, int NW>
 __device__ __forceinline__ void chain_pass_mfma(f32x4_t (&acc)[NS][NBW], const float* X,
                                                 const float* const (&wp)[NS], int nd,
                                                 int nblk, int wv, int lane, int Km) {
@@ -285,7 +282,7 @@ __device__ __forceinline__ void chain_pass_mfma(f32x4_t (&acc)[NS][NBW], const f
   // the loop); their results are never stored
   int rb[NBW];
 #pragma unroll
-  for (int i = 0; i < NBW; ++i) rb[i] = (min(wv + 4 * i, nblk - 1) * 16 + kk) * kChS + ii;
+  for (int i = 0; i < NBW; ++i) rb[i] = (min(wv + NW * i, nblk - 1) * 16 + kk) * kChS + ii;
   for (int d = 0; d < nd; ++d) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -306,7 +303,7 @@ __device__ __forceinline__ void chain_pass_mfma(f32x4_t (&acc)[NS][NBW], const f
 
 // The reference's arithmetic for the same outputs, in the same layout (the
 // taps outside the span are zero, so the same products in the same order).
-template <int NBW, int NS>
+template <int NBW, int NS, int NW>
 __device__ __forceinline__ void chain_pass_exact(f32x4_t (&acc)[NS][NBW], const float* X,
                                                  const float* const (&wp)[NS], int Km,
                                                  int nblk, int wv, int lane) {
@@ -315,7 +312,7 @@ __device__ __forceinline__ void chain_pass_exact(f32x4_t (&acc)[NS][NBW], const 
   for (int i = 0; i < NBW; ++i) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) acc[s][i] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-    const int b = wv + 4 * i;
+    const int b = wv + NW * i;
     if (b >= nblk) continue;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -333,14 +330,14 @@ __device__ __forceinline__ void chain_pass_exact(f32x4_t (&acc)[NS][NBW], const 
   }
 }
 
-template <int NBW, int NS>
+template <int NBW, int NS, int NW>
 __device__ __forceinline__ void chain_pass(bool exact, f32x4_t (&acc)[NS][NBW], const float* X,
                                            const float* const (&wp)[NS], int Km, int nd,
                                            int nblk, int wv, int lane) {
   if (exact)
-    chain_pass_exact<NBW, NS>(acc, X, wp, Km, nblk, wv, lane);
+    chain_pass_exact<NBW, NS, NW>(acc, X, wp, Km, nblk, wv, lane);
   else
-    chain_pass_mfma<NBW, NS>(acc, X, wp, nd, nblk, wv, lane, Km);
+    chain_pass_mfma<NBW, NS, NW>(acc, X, wp, nd, nblk, wv, lane, Km);
 }
 
 __device__ __forceinline__ float load_in(const void* in, int dt, int64_t g, bool negate) {
@@ -352,8 +349,8 @@ __device__ __forceinline__ float load_in(const void* in, int dt, int64_t g, bool
 // Zero-padded trimmed taps of both sets into LDS.
 __device__ __forceinline__ void stage_trimmed(float* wpS, float* wpD, const TapPack<float>& S,
                                               const TapPack<float>& D, int t0, int Km, int WP,
-                                              int tid) {
-  for (int e = tid; e < WP; e += 256) {
+                                              int tid, int nt) {
+  for (int e = tid; e < WP; e += nt) {
     const int t = e - 15;
     wpS[e] = (t >= 0 && t < Km) ? S.w[t0 + t] : 0.f;
     wpD[e] = (t >= 0 && t < Km) ? D.w[t0 + t] : 0.f;
@@ -363,21 +360,21 @@ __device__ __forceinline__ void stage_trimmed(float* wpS, float* wpD, const TapP
 // Source row of every halo row of the line buffer (rows of position outside
 // [0, N)): shift + the boundary-mapped position, or -1 for the constant.
 // The same for every column and buffer, so computed once per work-group.
-__device__ __forceinline__ void halo_table(int* hsrc, int E, int64_t N, int shift, int mode,
-                                           int tid) {
+__device__ __forceinline__ void halo_table(short* hsrc, int E, int64_t N, int shift, int mode,
+                                           int tid, int nt) {
   const int nh = E - (int)N;
-  for (int h = tid; h < nh; h += 256) {
+  for (int h = tid; h < nh; h += nt) {
     const int q = h < shift ? h : h + (int)N;
     const int64_t r = remap((int64_t)q - shift, N, mode);
-    hsrc[h] = r >= 0 ? (int)r + shift : -1;
+    hsrc[h] = (short)(r >= 0 ? (int)r + shift : -1);
   }
 }
 
 // Boundary-extend the halo rows of X (and Y) from the rows inside.
 __device__ __forceinline__ void fill_halo(float* X, float* Y, int E, int N, int shift,
-                                          const int* hsrc, float cval, int tid) {
+                                          const short* hsrc, float cval, int tid, int nt) {
   const int nh = E - N;
-  for (int e = tid; e < nh * kChC; e += 256) {
+  for (int e = tid; e < nh * kChC; e += nt) {
     const int h = e >> 4, c = e & 15;
     const int q = h < shift ? h : h + N;
     const int src = hsrc[h];
@@ -389,10 +386,10 @@ __device__ __forceinline__ void fill_halo(float* X, float* Y, int E, int N, int 
 // Stages (tile, job) per work-group of the matrix-core chain pass.
 constexpr int kChainSPW = 1;
 
-template <int NBW>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_chain_mfma(
+template <int NBW, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) void k_chain_mfma(
     const ChainArgs<float> a) {
-  constexpr int NPF = mfma_npf<NBW>();
+  constexpr int NPF = kMfmaNpf, NT = 64 * NW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int E = a.E, Nr = a.Nr, Km = a.Km, shift = a.shift;
   const int nd = (Km + 30) / 16;  // input blocks per output block
@@ -401,16 +398,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   float* bufB = bufA + E * kChS;
   float* wpS = bufB + E * kChS;
   float* wpD = wpS + WP;
-  int* hsrc = reinterpret_cast<int*>(wpD + WP);
+  short* hsrc = reinterpret_cast<short*>(wpD + WP);
   const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
   const int kk = lane >> 4, ii = lane & 15;
-  stage_trimmed(wpS, wpD, a.S, a.D, a.t0, Km, WP, tid);
-  halo_table(hsrc, E, a.N, shift, a.mode, tid);
+  stage_trimmed(wpS, wpD, a.S, a.D, a.t0, Km, WP, tid, NT);
+  halo_table(hsrc, E, a.N, shift, a.mode, tid, NT);
   const int64_t N = a.N, inner = a.inner, nlines = a.nlines;
   const int mode = a.mode, nblk = Nr / 16;
   const int64_t ntile = (nlines + kChC - 1) / kChC, nst = ntile * a.njobs;
   const int64_t G = gridDim.x;
-  // this thread stages line c = tid % 16, positions tid / 16 + 16 u
+  // this thread stages line c = tid % 16, positions tid / 16 + 4 NW u
   const int cs = tid & 15, ns0 = tid >> 4;
   auto load = [&](float (&v)[NPF], int64_t st) {
     const int job = (int)(st / ntile);
@@ -421,7 +418,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const int64_t base = lo * N * inner + li;
 #pragma unroll
     for (int u = 0; u < NPF; ++u) {
-      const int n = ns0 + 16 * u;
+      const int n = ns0 + 4 * NW * u;
       v[u] = (ok && n < N) ? load_in(jb.in, jb.in_dt, base + n * inner, a.negate != 0) : 0.f;
     }
   };
@@ -439,7 +436,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     bool bad = false;
 #pragma unroll
     for (int u = 0; u < NPF; ++u) {
-      const int n = ns0 + 16 * u;
+      const int n = ns0 + 4 * NW * u;
       if (n < N) {
         bufA[(shift + n) * kChS + cs] = pf[u];
         bad |= !finite(pf[u]);
@@ -448,7 +445,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const bool skip0 = __syncthreads_or(bad) || cval_bad;
     // the work-group's next stage is in flight during this one's passes
     if (st + 1 < nst && st + 1 < st0 + kChainSPW) load(pf, st + 1);
-    fill_halo(bufA, nullptr, E, (int)N, shift, hsrc, a.cval, tid);
+    fill_halo(bufA, nullptr, E, (int)N, shift, hsrc, a.cval, tid, NT);
     __syncthreads();
     bool need_s = false, need_d = false;
     for (int k = 0; k < jb.nout; ++k) {
@@ -459,11 +456,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     f32x4_t f[2][NBW];
     if (need_s && need_d) {
       const float* const wp[2] = {wpS, wpD};
-      chain_pass<NBW, 2>(skip0, f, bufA, wp, Km, nd, nblk, wv, lane);
+      chain_pass<NBW, 2, NW>(skip0, f, bufA, wp, Km, nd, nblk, wv, lane);
     } else {
       f32x4_t g[1][NBW];
       const float* const wp[1] = {need_s ? wpS : wpD};
-      chain_pass<NBW, 1>(skip0, g, bufA, wp, Km, nd, nblk, wv, lane);
+      chain_pass<NBW, 1, NW>(skip0, g, bufA, wp, Km, nd, nblk, wv, lane);
 #pragma unroll
       for (int i = 0; i < NBW; ++i) {
         f[0][i] = g[0][i];
@@ -474,7 +471,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     bool bad1 = false;
 #pragma unroll
     for (int i = 0; i < NBW; ++i) {
-      const int b = wv + 4 * i;
+      const int b = wv + NW * i;
       if (b >= nblk) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -485,7 +482,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       }
     }
     const bool skip1 = __syncthreads_or(bad1) || skip0;
-    fill_halo(bufA, bufB, E, (int)N, shift, hsrc, a.cval, tid);
+    fill_halo(bufA, bufB, E, (int)N, shift, hsrc, a.cval, tid, NT);
     __syncthreads();
     // second passes, per source buffer
     const int64_t L = L0 + ii;
@@ -504,11 +501,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       f32x4_t y[2][NBW];
       if (ks >= 0 && kd >= 0) {
         const float* const wp[2] = {wpS, wpD};
-        chain_pass<NBW, 2>(skip1, y, X, wp, Km, nd, nblk, wv, lane);
+        chain_pass<NBW, 2, NW>(skip1, y, X, wp, Km, nd, nblk, wv, lane);
       } else {
         f32x4_t g[1][NBW];
         const float* const wp[1] = {ks >= 0 ? wpS : wpD};
-        chain_pass<NBW, 1>(skip1, g, X, wp, Km, nd, nblk, wv, lane);
+        chain_pass<NBW, 1, NW>(skip1, g, X, wp, Km, nd, nblk, wv, lane);
 #pragma unroll
         for (int i = 0; i < NBW; ++i) {
           y[0][i] = g[0][i];
@@ -520,7 +517,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       if (!lok) continue;
 #pragma unroll
       for (int i = 0; i < NBW; ++i) {
-        const int b = wv + 4 * i;
+        const int b = wv + NW * i;
         if (b >= nblk) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -848,8 +845,8 @@ struct FinalGeom {
 };
 
 // Thread's share of a tile's interior: row c = u % 16, position
-// tid + 256 (u / 16).
-template <int NPF>
+// tid + 64 NW (u / 16).
+template <int NPF, int NW>
 __device__ __forceinline__ void final_load(const FinalArgs<float>& a, const FinalGeom& g,
                                            float (&v)[NPF], int J, int64_t tile, int tid) {
   const int64_t row0 = tile * kChC;
@@ -859,7 +856,7 @@ __device__ __forceinline__ void final_load(const FinalArgs<float>& a, const Fina
     // a wave-uniform row pointer and a 32-bit lane offset
     const int64_t row = row0 + (u & 15);
     const float* rp = a.in[J] + row * g.N;
-    const int q = tid + 256 * (u >> 4);
+    const int q = tid + 64 * NW * (u >> 4);
     v[u] = (row < g.nrows && q < n) ? rp[q] : 0.f;
   }
 }
@@ -870,21 +867,21 @@ constexpr bool kFinalPrefetch = false;
 
 // Input J of the tile: store the loaded input, start the next load, first
 // op, re-extend, second op into res.
-template <int NIN, int NBW, int J>
+template <int NIN, int NBW, int NW, int J>
 __device__ __forceinline__ void final_input_mfma(const FinalArgs<float>& a, const FinalGeom& g,
                                                  float* xin, float* xf, const float* wpS,
-                                                 const float* wpD, const int* hsrc, int64_t tile,
-                                                 float (&pf)[mfma_npf<NBW>()],
+                                                 const float* wpD, const short* hsrc,
+                                                 int64_t tile, float (&pf)[kMfmaNpf],
                                                  f32x4_t (&res)[NBW]) {
   const int E = g.E, shift = g.shift;
   const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
   const int kk = lane >> 4, ii = lane & 15;
-  constexpr int NPF = mfma_npf<NBW>();
-  if (!kFinalPrefetch) final_load<NPF>(a, g, pf, J, tile, tid);
+  constexpr int NPF = kMfmaNpf, NT = 64 * NW;
+  if (!kFinalPrefetch) final_load<NPF, NW>(a, g, pf, J, tile, tid);
   bool bad = false;
 #pragma unroll
   for (int u = 0; u < NPF; ++u) {
-    const int q = tid + 256 * (u >> 4);
+    const int q = tid + NT * (u >> 4);
     if (q < g.N) {
       xin[(shift + q) * kChS + (u & 15)] = pf[u];
       bad |= !finite(pf[u]);
@@ -892,30 +889,30 @@ __device__ __forceinline__ void final_input_mfma(const FinalArgs<float>& a, cons
   }
   const bool skip0 = __syncthreads_or(bad) || !finite(a.cval);
   if (kFinalPrefetch && J + 1 < NIN)
-    final_load<NPF>(a, g, pf, J + 1 < NIN ? J + 1 : 0, tile, tid);
-  fill_halo(xin, nullptr, E, (int)g.N, shift, hsrc, a.cval, tid);
+    final_load<NPF, NW>(a, g, pf, J + 1 < NIN ? J + 1 : 0, tile, tid);
+  fill_halo(xin, nullptr, E, (int)g.N, shift, hsrc, a.cval, tid, NT);
   __syncthreads();
   f32x4_t f[1][NBW];
   {
     const float* const wp[1] = {a.first[J] ? wpD : wpS};
-    chain_pass<NBW, 1>(skip0, f, xin, wp, g.Km, g.nd, g.nblk, wv, lane);
+    chain_pass<NBW, 1, NW>(skip0, f, xin, wp, g.Km, g.nd, g.nblk, wv, lane);
   }
   bool bad1 = false;
 #pragma unroll
   for (int i = 0; i < NBW; ++i) {
-    if (wv + 4 * i >= g.nblk) continue;
+    if (wv + NW * i >= g.nblk) continue;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      xf[(shift + 16 * (wv + 4 * i) + 4 * kk + r) * kChS + ii] = f[0][i][r];
+      xf[(shift + 16 * (wv + NW * i) + 4 * kk + r) * kChS + ii] = f[0][i][r];
       bad1 |= !finite(f[0][i][r]);
     }
   }
   const bool skip1 = __syncthreads_or(bad1) || skip0;
-  fill_halo(xf, nullptr, E, (int)g.N, shift, hsrc, a.cval, tid);
+  fill_halo(xf, nullptr, E, (int)g.N, shift, hsrc, a.cval, tid, NT);
   __syncthreads();
   {
     const float* const wp[1] = {a.second[J] ? wpD : wpS};
-    chain_pass<NBW, 1>(skip1, f, xf, wp, g.Km, g.nd, g.nblk, wv, lane);
+    chain_pass<NBW, 1, NW>(skip1, f, xf, wp, g.Km, g.nd, g.nblk, wv, lane);
   }
 #pragma unroll
   for (int i = 0; i < NBW; ++i) res[i] = f[0][i];
@@ -924,20 +921,21 @@ __device__ __forceinline__ void final_input_mfma(const FinalArgs<float>& a, cons
   // written after the first pass of the next input, behind that barrier too
 }
 
-template <int NIN, int NBW, size_t... Js>
+template <int NIN, int NBW, int NW, size_t... Js>
 __device__ __forceinline__ void final_all_mfma(const FinalArgs<float>& a, const FinalGeom& g,
                                                float* xin, float* xf, const float* wpS,
-                                               const float* wpD, const int* hsrc, int64_t tile,
-                                               float (&pf)[mfma_npf<NBW>()],
+                                               const float* wpD, const short* hsrc,
+                                               int64_t tile, float (&pf)[kMfmaNpf],
                                                f32x4_t (&res)[NIN][NBW],
                                                std::index_sequence<Js...>) {
-  (final_input_mfma<NIN, NBW, (int)Js>(a, g, xin, xf, wpS, wpD, hsrc, tile, pf, res[Js]), ...);
+  (final_input_mfma<NIN, NBW, NW, (int)Js>(a, g, xin, xf, wpS, wpD, hsrc, tile, pf, res[Js]),
+   ...);
 }
 
-template <int NIN, int NBW>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_final_mfma(
+template <int NIN, int NBW, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) void k_final_mfma(
     const FinalArgs<float> a) {
-  constexpr int NPF = mfma_npf<NBW>();
+  constexpr int NPF = kMfmaNpf, NT = 64 * NW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   FinalGeom g;
   g.E = a.E;
@@ -954,21 +952,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   float* xf = xin + g.E * kChS;
   float* wpS = xf + g.E * kChS;
   float* wpD = wpS + WP;
-  int* hsrc = reinterpret_cast<int*>(wpD + WP);
+  short* hsrc = reinterpret_cast<short*>(wpD + WP);
   const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
   const int kk = lane >> 4, ii = lane & 15;
-  stage_trimmed(wpS, wpD, a.S, a.D, a.t0, a.Km, WP, tid);
-  halo_table(hsrc, g.E, g.N, g.shift, a.mode, tid);
+  stage_trimmed(wpS, wpD, a.S, a.D, a.t0, a.Km, WP, tid, NT);
+  halo_table(hsrc, g.E, g.N, g.shift, a.mode, tid, NT);
   double gsq = a.gamma_sq;
   if (a.epi == EPI_FRANGI && a.gamma_sq_dev) gsq = *a.gamma_sq_dev;
   const int64_t N = g.N;
   float rgmax = 0.f;
   float pf[NPF];
   const int64_t tile = xcd_swizzle(blockIdx.x, g.G);
-  if (kFinalPrefetch) final_load<NPF>(a, g, pf, 0, tile, tid);
+  if (kFinalPrefetch) final_load<NPF, NW>(a, g, pf, 0, tile, tid);
   {
     f32x4_t res[NIN][NBW];
-    final_all_mfma<NIN, NBW>(a, g, xin, xf, wpS, wpD, hsrc, tile, pf, res,
+    final_all_mfma<NIN, NBW, NW>(a, g, xin, xf, wpS, wpD, hsrc, tile, pf, res,
                              std::make_index_sequence<NIN>{});
     // epilogue (k_final_contig's)
     const int64_t row = tile * kChC + ii;
@@ -976,8 +974,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
       for (int v = 0; v < 4 * NBW; ++v) {
         const int i = v >> 2;
-        if (wv + 4 * i >= g.nblk) break;
-        const int64_t p = 16 * (wv + 4 * i) + 4 * kk + (v & 3);
+        if (wv + NW * i >= g.nblk) break;
+        const int64_t p = 16 * (wv + NW * i) + 4 * kk + (v & 3);
         if (p >= N) continue;
         float h[NIN];
 #pragma unroll
@@ -1178,7 +1176,7 @@ struct MfmaGeomHost {
   int t0, Km, shift, Nr, E, nd, lds;
 };
 
-inline bool mfma_geom(const TapPack<float>& S, const TapPack<float>& D, int64_t N, int npf,
+inline bool mfma_geom(const TapPack<float>& S, const TapPack<float>& D, int64_t N,
                       MfmaGeomHost& g) {
   if (!chain_mfma_enabled() || N < 1 || N > 512) return false;
   int lo = S.K, hi = -1;
@@ -1196,39 +1194,46 @@ inline bool mfma_geom(const TapPack<float>& S, const TapPack<float>& D, int64_t 
   // the last input block reads row Nr + 16 nd - 17 <= Nr + Km + 13
   g.E = g.Nr + g.Km + 14;
   g.nd = (g.Km + 30) / 16;
-  g.lds = (2 * g.E * kChS + 2 * (16 * g.nd + 16) + g.E) * (int)sizeof(float);
-  return g.lds <= kMaxLds && N <= 16 * npf;
+  // two line buffers, the padded taps, the halo source rows (16-bit)
+  g.lds = (2 * g.E * kChS + 2 * (16 * g.nd + 16)) * (int)sizeof(float) + (g.E + 1) / 2 * 4;
+  return g.lds <= kMaxLds;
 }
 
-template <int NBW>
+// Waves per work-group of the matrix-core passes: 4 for lines of at most
+// 256, 8 up to 512 (each thread stages kMfmaNpf values of a tile's interior,
+// and a wave holds at most 4 blocks of 16 outputs).
+inline int mfma_waves(int64_t N) { return N <= 64 * 4 ? 4 : 8; }
+
+template <int NBW, int NW>
 int launch_chain_mfma_nbw(ChainArgs<float>& a, const MfmaGeomHost& g, hipStream_t st) {
-  const void* f = (const void*)k_chain_mfma<NBW>;
+  const void* f = (const void*)k_chain_mfma<NBW, NW>;
   if (g.lds > 64 * 1024) {
     const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, g.lds);
     if (e != hipSuccess) return hip_check(e, "hipFuncSetAttribute(k_chain_mfma)");
   }
   const int64_t work = (a.nlines + kChC - 1) / kChC * a.njobs;
   const int64_t G = (work + kChainSPW - 1) / kChainSPW;
-  HC_LAUNCH("k_chain_mfma", st, (k_chain_mfma<NBW>), dim3((unsigned)G), dim3(256), g.lds, a);
+  HC_LAUNCH("k_chain_mfma", st, (k_chain_mfma<NBW, NW>), dim3((unsigned)G), dim3(64 * NW), g.lds,
+            a);
   HC_HIP(hipGetLastError());
   return HC_OK;
 }
 
-// Matrix-core strided chain pass (float32, lines of at most 256: longer
-// ones would need more registers than a lane has); returns -1 when it does
-// not apply.
+// Matrix-core strided chain pass (float32, lines of at most 512); returns -1
+// when it does not apply.
 inline int launch_chain_mfma(ChainArgs<float>& a, hipStream_t st) {
   const int nblk = (int)((a.N + 15) / 16);
   MfmaGeomHost g;
-  if (nblk > 16 || !mfma_geom(a.S, a.D, a.N, mfma_npf<4>(), g)) return -1;
+  if (!mfma_geom(a.S, a.D, a.N, g)) return -1;
   a.t0 = g.t0;
   a.Km = g.Km;
   a.shift = g.shift;
   a.Nr = g.Nr;
   a.E = g.E;
-  if (nblk <= 4) return launch_chain_mfma_nbw<1>(a, g, st);
-  if (nblk <= 8) return launch_chain_mfma_nbw<2>(a, g, st);
-  return launch_chain_mfma_nbw<4>(a, g, st);
+  if (mfma_waves(a.N) == 8) return launch_chain_mfma_nbw<4, 8>(a, g, st);
+  if (nblk <= 4) return launch_chain_mfma_nbw<1, 4>(a, g, st);
+  if (nblk <= 8) return launch_chain_mfma_nbw<2, 4>(a, g, st);
+  return launch_chain_mfma_nbw<4, 4>(a, g, st);
 }
 
 template <typename W>
@@ -1303,40 +1308,37 @@ int prepare_final(FinalArgs<W>* a, const hc_array* shape_src, const AxisTaps<W>&
   return HC_OK;
 }
 
-template <int NIN, int NBW>
+template <int NIN, int NBW, int NW>
 int launch_final_mfma_nbw(const FinalArgs<float>& a, const MfmaGeomHost& g, hipStream_t st) {
-  const void* f = (const void*)k_final_mfma<NIN, NBW>;
+  const void* f = (const void*)k_final_mfma<NIN, NBW, NW>;
   if (g.lds > 64 * 1024) {
     const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, g.lds);
     if (e != hipSuccess) return hip_check(e, "hipFuncSetAttribute(k_final_mfma)");
   }
   const int64_t G = (a.nrows + kChC - 1) / kChC;
-  HC_LAUNCH("k_final_mfma", st, (k_final_mfma<NIN, NBW>), dim3((unsigned)G), dim3(256), g.lds, a);
+  HC_LAUNCH("k_final_mfma", st, (k_final_mfma<NIN, NBW, NW>), dim3((unsigned)G), dim3(64 * NW),
+            g.lds, a);
   HC_HIP(hipGetLastError());
   return HC_OK;
 }
 
-// Matrix-core final pass for float32 rows of at most 256 voxels (3-D: six
-// Hessian elements of 16 voxels per lane) or 512 (2-D); returns -1 when it
-// does not apply.
+// Matrix-core final pass for float32 rows of at most 512 voxels; returns -1
+// when it does not apply.
 template <int NIN>
 int launch_final_mfma(const FinalArgs<float>& a0, hipStream_t st) {
-  constexpr int kMaxN = NIN == 6 ? 256 : 512;
-  if (a0.N > kMaxN) return -1;
   const int nblk = (int)((a0.N + 15) / 16);
   MfmaGeomHost g;
-  if (!mfma_geom(a0.S, a0.D, a0.N, nblk <= 16 ? mfma_npf<4>() : mfma_npf<8>(), g)) return -1;
+  if (!mfma_geom(a0.S, a0.D, a0.N, g)) return -1;
   FinalArgs<float> a = a0;
   a.t0 = g.t0;
   a.Km = g.Km;
   a.shift = g.shift;
   a.Nr = g.Nr;
   a.E = g.E;
-  constexpr int kTop = NIN == 6 ? 4 : 8;
-  if (nblk <= 4) return launch_final_mfma_nbw<NIN, 1>(a, g, st);
-  if (nblk <= 8) return launch_final_mfma_nbw<NIN, 2>(a, g, st);
-  if (nblk <= 16) return launch_final_mfma_nbw<NIN, 4>(a, g, st);
-  return launch_final_mfma_nbw<NIN, kTop>(a, g, st);
+  if (mfma_waves(a.N) == 8) return launch_final_mfma_nbw<NIN, 4, 8>(a, g, st);
+  if (nblk <= 4) return launch_final_mfma_nbw<NIN, 1, 4>(a, g, st);
+  if (nblk <= 8) return launch_final_mfma_nbw<NIN, 2, 4>(a, g, st);
+  return launch_final_mfma_nbw<NIN, 4, 4>(a, g, st);
 }
 
 template <typename W>

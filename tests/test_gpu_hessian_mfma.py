@@ -1,8 +1,8 @@
 """Matrix-core (v_mfma_f32_16x16x4f32) Hessian passes.
 
-The float32 passes of the fused Hessian (strided line passes for lines of up
-to 256 voxels, the final pass along the contiguous axis for rows of up to 256
-voxels in 3-D and 512 in 2-D) run on the matrix cores by default; the
+The float32 passes of the fused Hessian (strided line passes and the final
+pass along the contiguous axis, for lines of up to 512 voxels: 4 waves per
+work-group up to 256, 8 above) run on the matrix cores by default; the
 HC_TUNE_HESSIAN_VALU knob switches them to the vector-ALU passes, tap by tap
 as the reference computes them.  Tolerances (float32 data):
 Hessian elements within 2e-6 * max|H| of the FP64 oracle and of the VALU
@@ -55,7 +55,7 @@ def _hessian(x, sigma, mode, valu):
 
 @pytest.mark.parametrize("shape", [(512, 24), (509, 40), (33, 17), (16, 16), (1, 40), (24, 512),
                                    (17, 300), (20, 130, 9), (64, 48, 20), (9, 20, 256),
-                                   (5, 7, 300)])
+                                   (5, 7, 300), (300, 9, 20), (6, 511, 17), (4, 5, 512)])
 @pytest.mark.parametrize("sigma", [1.0, 2.5, 5.0])
 @pytest.mark.parametrize("mode", ["reflect", "constant", "wrap"])
 def test_mfma_hessian_vs_fp64_and_valu(gpu, shape, sigma, mode):
