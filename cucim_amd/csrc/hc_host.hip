@@ -139,8 +139,9 @@ int hc_tune_set(int knob, int value) {
     return hc::fail(HC_ERR_VALUE, "segment must be a multiple of 256 in [256, 4096]");
   if (knob == HC_TUNE_HESSIAN_VALU && value != 0 && value != 1)
     return hc::fail(HC_ERR_VALUE, "the Hessian vector-ALU switch takes 0 or 1");
-  if (knob == HC_TUNE_SEP2D_TY && value != 0 && value != 16 && value != 32 && value != 64)
-    return hc::fail(HC_ERR_VALUE, "tile rows must be 16, 32 or 64");
+  if (knob == HC_TUNE_SEP2D_TY && value != 0 && value != 8 && value != 16 && value != 32 &&
+      value != 64)
+    return hc::fail(HC_ERR_VALUE, "tile rows must be 8, 16, 32 or 64");
   hc::g_tune[knob].store(value);
   return HC_OK;
 }

@@ -104,10 +104,12 @@ int launch_reg(Reg2dArgs a, const hc_taps* t0, const hc_taps* t1, hipStream_t st
 
 template <int K>
 int launch_reg_ty(const Reg2dArgs& a, const hc_taps* t0, const hc_taps* t1, hipStream_t st) {
-  // 16 rows measure faster than 32 at 2048^2 .. 8192^2 for K = 13 .. 33
-  // (more resident work-groups outweigh the taller tile's smaller halo)
+  // Default: 8 rows up to 4 Mpixel (2048^2: more work-groups for a launch
+  // that is one wave of the chip, 18 vs 19 us), 16 above (8, 16 and 32 rows
+  // measure within noise of each other at 8192^2)
   int ty = tune_get(HC_TUNE_SEP2D_TY);
-  if (ty != 16 && ty != 32) ty = 16;
+  if (ty != 8 && ty != 16 && ty != 32) ty = a.H * a.W <= (int64_t(1) << 22) ? 8 : 16;
+  if (ty == 8) return launch_reg<K, 8>(a, t0, t1, st);
   return ty == 32 ? launch_reg<K, 32>(a, t0, t1, st) : launch_reg<K, 16>(a, t0, t1, st);
 }
 

@@ -106,7 +106,8 @@ typedef enum hc_tune_knob {
   HC_TUNE_CONTIG_SEG = 0,   /* contiguous-axis correlate: outputs per wave
                                segment, 256..4096, multiple of 256 */
   HC_TUNE_SEP2D_TY = 1,     /* fused 2-D separable: output rows per tile,
-                               16, 32 or 64 */
+                               8 (float32 register-blocked kernel only), 16,
+                               32 or 64 */
   HC_TUNE_HESSIAN_VALU = 2, /* 1: fused Hessian passes on the vector ALUs
                                only, tap by tap as the reference computes
                                (default 0: float32 passes on the matrix

@@ -300,7 +300,7 @@ def test_tuned_contig_segment(gpu, seg):
         lib.hc_tune_set(_lib.TUNE_CONTIG_SEG, 0)
 
 
-@pytest.mark.parametrize("ty", [16, 32, 64])
+@pytest.mark.parametrize("ty", [8, 16, 32, 64])
 @pytest.mark.parametrize("shape", [(300, 260), (257, 131)])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_tuned_sep2d_rows(gpu, ty, shape, dtype, monkeypatch):
