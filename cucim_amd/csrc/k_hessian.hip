@@ -248,6 +248,11 @@ __global__ __launch_bounds__(256) void k_chain_strided(const ChainArgs<W> a) {
 // in registers while the current one is computed.
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
+// Row stride (floats) of the matrix-core passes' line buffers: 16, so that a
+// wave's B fragment (4 rows x 16 lines) is 64 consecutive words (no bank
+// conflict) and two buffers of a 256-voxel line fit four work-groups per CU.
+constexpr int kMfS = 16;
+
 __device__ __forceinline__ f32x4_t mfma4(float a, float b, f32x4_t c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
@@ -282,7 +287,7 @@ __device__ __forceinline__ void chain_pass_mfma(f32x4_t (&acc)[NS][NBW], const f
   // the loop); their results are never stored
   int rb[NBW];
 #pragma unroll
-  for (int i = 0; i < NBW; ++i) rb[i] = (min(wv + NW * i, nblk - 1) * 16 + kk) * kChS + ii;
+  for (int i = 0; i < NBW; ++i) rb[i] = (min(wv + NW * i, nblk - 1) * 16 + kk) * kMfS + ii;
   for (int d = 0; d < nd; ++d) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -293,7 +298,7 @@ __device__ __forceinline__ void chain_pass_mfma(f32x4_t (&acc)[NS][NBW], const f
       for (int s = 0; s < NS; ++s) av[s] = wp[s][wi];
 #pragma unroll
       for (int i = 0; i < NBW; ++i) {
-        const float bx = X[rb[i] + (16 * d + 4 * u) * kChS];
+        const float bx = X[rb[i] + (16 * d + 4 * u) * kMfS];
 #pragma unroll
         for (int s = 0; s < NS; ++s) acc[s][i] = mfma4(av[s], bx, acc[s][i]);
       }
@@ -322,7 +327,7 @@ __device__ __forceinline__ void chain_pass_exact(f32x4_t (&acc)[NS][NBW], const 
         float v = 0.f;
         for (int t = 0; t < Km; ++t) {
           const float w = wp[s][15 + t];
-          if (w != 0.f) v = fmaf(X[(p + t) * kChS + ii], w, v);
+          if (w != 0.f) v = fmaf(X[(p + t) * kMfS + ii], w, v);
         }
         acc[s][i][r] = v;
       }
@@ -378,8 +383,8 @@ __device__ __forceinline__ void fill_halo(float* X, float* Y, int E, int N, int 
     const int h = e >> 4, c = e & 15;
     const int q = h < shift ? h : h + N;
     const int src = hsrc[h];
-    X[q * kChS + c] = src >= 0 ? X[src * kChS + c] : cval;
-    if (Y) Y[q * kChS + c] = src >= 0 ? Y[src * kChS + c] : cval;
+    X[q * kMfS + c] = src >= 0 ? X[src * kMfS + c] : cval;
+    if (Y) Y[q * kMfS + c] = src >= 0 ? Y[src * kMfS + c] : cval;
   }
 }
 
@@ -395,8 +400,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) vo
   const int nd = (Km + 30) / 16;  // input blocks per output block
   const int WP = 16 * nd + 16;    // padded tap array length
   float* bufA = reinterpret_cast<float*>(smem_raw);
-  float* bufB = bufA + E * kChS;
-  float* wpS = bufB + E * kChS;
+  float* bufB = bufA + E * kMfS;
+  float* wpS = bufB + E * kMfS;
   float* wpD = wpS + WP;
   short* hsrc = reinterpret_cast<short*>(wpD + WP);
   const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
@@ -438,7 +443,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) vo
     for (int u = 0; u < NPF; ++u) {
       const int n = ns0 + 4 * NW * u;
       if (n < N) {
-        bufA[(shift + n) * kChS + cs] = pf[u];
+        bufA[(shift + n) * kMfS + cs] = pf[u];
         bad |= !finite(pf[u]);
       }
     }
@@ -476,8 +481,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) vo
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int q = shift + 16 * b + 4 * kk + r;
-        bufA[q * kChS + ii] = f[0][i][r];
-        bufB[q * kChS + ii] = f[1][i][r];
+        bufA[q * kMfS + ii] = f[0][i][r];
+        bufB[q * kMfS + ii] = f[1][i][r];
         bad1 |= !(finite(f[0][i][r]) && finite(f[1][i][r]));
       }
     }
@@ -883,7 +888,7 @@ __device__ __forceinline__ void final_input_mfma(const FinalArgs<float>& a, cons
   for (int u = 0; u < NPF; ++u) {
     const int q = tid + NT * (u >> 4);
     if (q < g.N) {
-      xin[(shift + q) * kChS + (u & 15)] = pf[u];
+      xin[(shift + q) * kMfS + (u & 15)] = pf[u];
       bad |= !finite(pf[u]);
     }
   }
@@ -903,7 +908,7 @@ __device__ __forceinline__ void final_input_mfma(const FinalArgs<float>& a, cons
     if (wv + NW * i >= g.nblk) continue;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      xf[(shift + 16 * (wv + NW * i) + 4 * kk + r) * kChS + ii] = f[0][i][r];
+      xf[(shift + 16 * (wv + NW * i) + 4 * kk + r) * kMfS + ii] = f[0][i][r];
       bad1 |= !finite(f[0][i][r]);
     }
   }
@@ -933,7 +938,7 @@ __device__ __forceinline__ void final_all_mfma(const FinalArgs<float>& a, const 
 }
 
 template <int NIN, int NBW, int NW>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) void k_final_mfma(
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void k_final_mfma(
     const FinalArgs<float> a) {
   constexpr int NPF = kMfmaNpf, NT = 64 * NW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -949,8 +954,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) vo
   g.G = gridDim.x;
   const int WP = 16 * g.nd + 16;
   float* xin = reinterpret_cast<float*>(smem_raw);
-  float* xf = xin + g.E * kChS;
-  float* wpS = xf + g.E * kChS;
+  float* xf = xin + g.E * kMfS;
+  float* wpS = xf + g.E * kMfS;
   float* wpD = wpS + WP;
   short* hsrc = reinterpret_cast<short*>(wpD + WP);
   const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
@@ -1195,7 +1200,7 @@ inline bool mfma_geom(const TapPack<float>& S, const TapPack<float>& D, int64_t 
   g.E = g.Nr + g.Km + 14;
   g.nd = (g.Km + 30) / 16;
   // two line buffers, the padded taps, the halo source rows (16-bit)
-  g.lds = (2 * g.E * kChS + 2 * (16 * g.nd + 16)) * (int)sizeof(float) + (g.E + 1) / 2 * 4;
+  g.lds = (2 * g.E * kMfS + 2 * (16 * g.nd + 16)) * (int)sizeof(float) + (g.E + 1) / 2 * 4;
   return g.lds <= kMaxLds;
 }
 
