@@ -253,6 +253,10 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 // conflict) and two buffers of a 256-voxel line fit four work-groups per CU.
 constexpr int kMfS = 16;
 
+// Length of the zero-padded tap arrays: the passes read wp[15 + 4 s + k - i]
+// for steps s < ceil((Km + 15) / 4), k < 4, i < 16.
+__host__ __device__ constexpr int mfma_wp(int Km) { return 4 * ((Km + 18) / 4) + 16; }
+
 __device__ __forceinline__ f32x4_t mfma4(float a, float b, f32x4_t c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
@@ -398,7 +402,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) vo
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int E = a.E, Nr = a.Nr, Km = a.Km, shift = a.shift;
   const int nd = (Km + 30) / 16;  // input blocks per output block
-  const int WP = 16 * nd + 16;    // padded tap array length
+  const int WP = mfma_wp(Km);     // padded tap array length
   float* bufA = reinterpret_cast<float*>(smem_raw);
   float* bufB = bufA + E * kMfS;
   float* wpS = bufB + E * kMfS;
@@ -952,7 +956,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
   g.nrows = a.nrows;
   g.ntile = (a.nrows + kChC - 1) / kChC;
   g.G = gridDim.x;
-  const int WP = 16 * g.nd + 16;
+  const int WP = mfma_wp(g.Km);
   float* xin = reinterpret_cast<float*>(smem_raw);
   float* xf = xin + g.E * kMfS;
   float* wpS = xf + g.E * kMfS;
@@ -1196,11 +1200,16 @@ inline bool mfma_geom(const TapPack<float>& S, const TapPack<float>& D, int64_t 
   g.shift = S.anchor - lo;
   if (g.shift < 0 || g.shift > g.Km + 14) return false;
   g.Nr = (int)((N + 15) / 16 * 16);
-  // the last input block reads row Nr + 16 nd - 17 <= Nr + Km + 13
-  g.E = g.Nr + g.Km + 14;
+  // rows the passes touch: the matrix-core steps read up to row
+  // Nr - 17 + 4 ceil((Km + 15) / 4), the tap-by-tap path up to Nr + Km - 2,
+  // the first-pass results go to rows shift .. shift + Nr - 1
+  const int nsteps = (g.Km + 18) / 4;
+  g.E = std::max(std::max(g.Nr + 4 * nsteps - 16, g.Nr + g.Km - 1), g.shift + g.Nr);
   g.nd = (g.Km + 30) / 16;
-  // two line buffers, the padded taps, the halo source rows (16-bit)
-  g.lds = (2 * g.E * kMfS + 2 * (16 * g.nd + 16)) * (int)sizeof(float) + (g.E + 1) / 2 * 4;
+  // two line buffers, the padded taps, the halo source rows (16-bit, one
+  // per row outside [shift, shift + N))
+  g.lds = (2 * g.E * kMfS + 2 * mfma_wp(g.Km)) * (int)sizeof(float) +
+          ((int)(g.E - N) + 1) / 2 * 4;
   return g.lds <= kMaxLds;
 }
 
