@@ -396,16 +396,15 @@ __device__ __forceinline__ void fill_halo(float* X, float* Y, int E, int N, int 
 constexpr int kChainSPW = 1;
 
 template <int NBW, int NW>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) void k_chain_mfma(
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(5))) void k_chain_mfma(
     const ChainArgs<float> a) {
   constexpr int NPF = kMfmaNpf, NT = 64 * NW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int E = a.E, Nr = a.Nr, Km = a.Km, shift = a.shift;
   const int nd = (Km + 30) / 16;  // input blocks per output block
   const int WP = mfma_wp(Km);     // padded tap array length
-  float* bufA = reinterpret_cast<float*>(smem_raw);
-  float* bufB = bufA + E * kMfS;
-  float* wpS = bufB + E * kMfS;
+  float* bufA = reinterpret_cast<float*>(smem_raw);  // the one line buffer
+  float* wpS = bufA + E * kMfS;
   float* wpD = wpS + WP;
   short* hsrc = reinterpret_cast<short*>(wpD + WP);
   const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
@@ -480,25 +479,18 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) vo
     bool bad1 = false;
 #pragma unroll
     for (int i = 0; i < NBW; ++i) {
-      const int b = wv + NW * i;
-      if (b >= nblk) continue;
+      if (wv + NW * i >= nblk) continue;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int q = shift + 16 * b + 4 * kk + r;
-        bufA[q * kMfS + ii] = f[0][i][r];
-        bufB[q * kMfS + ii] = f[1][i][r];
-        bad1 |= !(finite(f[0][i][r]) && finite(f[1][i][r]));
-      }
+      for (int r = 0; r < 4; ++r) bad1 |= !(finite(f[0][i][r]) && finite(f[1][i][r]));
     }
-    const bool skip1 = __syncthreads_or(bad1) || skip0;
-    fill_halo(bufA, bufB, E, (int)N, shift, hsrc, a.cval, tid, NT);
-    __syncthreads();
-    // second passes, per source buffer
     const int64_t L = L0 + ii;
     const bool lok = L < nlines;
     const int64_t lo = lok ? L / inner : 0, li = lok ? L - lo * inner : 0;
     const int64_t base = lo * N * inner + li;
-#pragma unroll 1
+    // second passes, per first op: its results go through the one line
+    // buffer in turn (the other op's results wait in registers)
+    bool skip1 = skip0, first_src = true;
+#pragma unroll
     for (int src = 0; src < 2; ++src) {
       int ks = -1, kd = -1;
       for (int k = 0; k < jb.nout; ++k) {
@@ -506,15 +498,29 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) vo
         if (jb.o[k].second == 0) ks = k; else kd = k;
       }
       if (ks < 0 && kd < 0) continue;
-      const float* X = src ? bufB : bufA;
+      if (!first_src) __syncthreads();  // every read of the previous results is done
+#pragma unroll
+      for (int i = 0; i < NBW; ++i) {
+        const int b = wv + NW * i;
+        if (b >= nblk) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bufA[(shift + 16 * b + 4 * kk + r) * kMfS + ii] = f[src][i][r];
+      }
+      if (first_src)
+        skip1 = __syncthreads_or(bad1) || skip0;
+      else
+        __syncthreads();
+      first_src = false;
+      fill_halo(bufA, nullptr, E, (int)N, shift, hsrc, a.cval, tid, NT);
+      __syncthreads();
       f32x4_t y[2][NBW];
       if (ks >= 0 && kd >= 0) {
         const float* const wp[2] = {wpS, wpD};
-        chain_pass<NBW, 2, NW>(skip1, y, X, wp, Km, nd, nblk, wv, lane);
+        chain_pass<NBW, 2, NW>(skip1, y, bufA, wp, Km, nd, nblk, wv, lane);
       } else {
         f32x4_t g[1][NBW];
         const float* const wp[1] = {ks >= 0 ? wpS : wpD};
-        chain_pass<NBW, 1, NW>(skip1, g, X, wp, Km, nd, nblk, wv, lane);
+        chain_pass<NBW, 1, NW>(skip1, g, bufA, wp, Km, nd, nblk, wv, lane);
 #pragma unroll
         for (int i = 0; i < NBW; ++i) {
           y[0][i] = g[0][i];
@@ -523,17 +529,18 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) vo
       }
       float* os = ks >= 0 ? reinterpret_cast<float*>(jb.o[ks].out) : nullptr;
       float* od = kd >= 0 ? reinterpret_cast<float*>(jb.o[kd].out) : nullptr;
-      if (!lok) continue;
+      if (lok) {
 #pragma unroll
-      for (int i = 0; i < NBW; ++i) {
-        const int b = wv + NW * i;
-        if (b >= nblk) continue;
+        for (int i = 0; i < NBW; ++i) {
+          const int b = wv + NW * i;
+          if (b >= nblk) continue;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int64_t n = 16 * b + 4 * kk + r;
-          if (n < N) {
-            if (os) os[base + n * inner] = y[0][i][r];
-            if (od) od[base + n * inner] = y[1][i][r];
+          for (int r = 0; r < 4; ++r) {
+            const int64_t n = 16 * b + 4 * kk + r;
+            if (n < N) {
+              if (os) os[base + n * inner] = y[0][i][r];
+              if (od) od[base + n * inner] = y[1][i][r];
+            }
           }
         }
       }
@@ -1185,7 +1192,7 @@ struct MfmaGeomHost {
   int t0, Km, shift, Nr, E, nd, lds;
 };
 
-inline bool mfma_geom(const TapPack<float>& S, const TapPack<float>& D, int64_t N,
+inline bool mfma_geom(const TapPack<float>& S, const TapPack<float>& D, int64_t N, int nbuf,
                       MfmaGeomHost& g) {
   if (!chain_mfma_enabled() || N < 1 || N > 512) return false;
   int lo = S.K, hi = -1;
@@ -1206,9 +1213,9 @@ inline bool mfma_geom(const TapPack<float>& S, const TapPack<float>& D, int64_t 
   const int nsteps = (g.Km + 18) / 4;
   g.E = std::max(std::max(g.Nr + 4 * nsteps - 16, g.Nr + g.Km - 1), g.shift + g.Nr);
   g.nd = (g.Km + 30) / 16;
-  // two line buffers, the padded taps, the halo source rows (16-bit, one
+  // nbuf line buffers, the padded taps, the halo source rows (16-bit, one
   // per row outside [shift, shift + N))
-  g.lds = (2 * g.E * kMfS + 2 * mfma_wp(g.Km)) * (int)sizeof(float) +
+  g.lds = (nbuf * g.E * kMfS + 2 * mfma_wp(g.Km)) * (int)sizeof(float) +
           ((int)(g.E - N) + 1) / 2 * 4;
   return g.lds <= kMaxLds;
 }
@@ -1238,7 +1245,7 @@ int launch_chain_mfma_nbw(ChainArgs<float>& a, const MfmaGeomHost& g, hipStream_
 inline int launch_chain_mfma(ChainArgs<float>& a, hipStream_t st) {
   const int nblk = (int)((a.N + 15) / 16);
   MfmaGeomHost g;
-  if (!mfma_geom(a.S, a.D, a.N, g)) return -1;
+  if (!mfma_geom(a.S, a.D, a.N, 1, g)) return -1;
   a.t0 = g.t0;
   a.Km = g.Km;
   a.shift = g.shift;
@@ -1342,7 +1349,7 @@ template <int NIN>
 int launch_final_mfma(const FinalArgs<float>& a0, hipStream_t st) {
   const int nblk = (int)((a0.N + 15) / 16);
   MfmaGeomHost g;
-  if (!mfma_geom(a0.S, a0.D, a0.N, g)) return -1;
+  if (!mfma_geom(a0.S, a0.D, a0.N, 2, g)) return -1;
   FinalArgs<float> a = a0;
   a.t0 = g.t0;
   a.Km = g.Km;
