@@ -73,6 +73,24 @@ def test_mfma_hessian_vs_fp64_and_valu(gpu, shape, sigma, mode):
         assert np.abs(g - r).max() <= 2e-6 * scale + 1e-7, (np.abs(g - r).max(), scale)
 
 
+@pytest.mark.parametrize("shape", [(40, 33), (5, 300), (3, 17, 260), (21, 9, 7)])
+@pytest.mark.parametrize("sigma", [1.0, 3.0, 5.0])
+@pytest.mark.parametrize("mode", ["nearest", "mirror"])
+def test_mfma_hessian_other_modes(gpu, shape, sigma, mode):
+    """Boundary modes whose halo rows repeat one voxel (nearest) or skip the
+    edge voxel (mirror), on lines shorter than the kernel reach."""
+    import scipy.ndimage as sndi
+
+    rng = np.random.default_rng(14)
+    x = rng.standard_normal(shape).astype(np.float32)
+    got = _hessian(x, sigma, mode, valu=False)
+    exp = osk.hessian_matrix(x.astype(np.float64), sigma, mode=mode, cval=0.0,
+                             use_gaussian_derivatives=True, ndi=sndi)
+    for g, e in zip(got, exp):
+        scale = np.abs(e).max()
+        assert np.abs(g - e).max() <= 2e-6 * scale + 1e-7, (np.abs(g - e).max(), scale)
+
+
 @pytest.mark.parametrize("value", [np.inf, -np.inf, np.nan])
 def test_mfma_non_finite_pattern(gpu, value):
     """0 * inf is NaN on the matrix cores: a group with a non-finite input
