@@ -28,7 +28,7 @@ SIGMAS = (1.0, 2.0, 3.0, 4.0, 5.0)
 SHAPE = (256, 256, 256)
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3      # published MI355X FP32
-FP32_VALU_NOPACK_TFLOPS = 78.6  # without packed FP32 (disabled in this build)
+FP32_VALU_NOPACK_TFLOPS = 78.6  # without packed FP32 (disabled at compile time)
 
 
 def _parse():
@@ -519,7 +519,7 @@ def _corr_roofline(corr):
 def main():
     args = _parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     if args.summarize:
@@ -608,8 +608,8 @@ def main():
                      "output, the matrix cores also multiply the zero band of each "
                      "16-output block (4*ceil((K+15)/4) products per output)")
                     if on_mfma else
-                    ("FP32 FMA sweeps of the separable passes; packed FP32 is disabled in "
-                     "this build, so the reachable VALU ceiling is "
+                    ("FP32 FMA sweeps of the separable passes; packed FP32 is disabled at "
+                     "compile time, so the reachable VALU ceiling is "
                      f"{FP32_VALU_NOPACK_TFLOPS} TFLOP/s"),
             "avg_launch_ms": avg_ms,
             "launches_per_step": cnt / args.steps,

@@ -2,8 +2,8 @@
 //
 // The network (5 layers of 3x3x3 "valid" convolutions, 24 channels, GELU
 // between layers) has no counterpart in the reference; it stands in for
-// filters/ridges.py:322-476 (python/cucim/src/cucim/skimage/) and is judged
-// against the plain kernels.
+// filters/ridges.py:322-476 (python/cucim/src/cucim/skimage/) and its output
+// is compared with the plain kernels'.
 //
 // Activations are channels-last bfloat16 volumes (D, H, W, C); weights are
 // bfloat16, accumulation float32 on the matrix cores:

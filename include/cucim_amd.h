@@ -236,8 +236,8 @@ int hc_meijering_fold(const hc_array* H, int nH, const hc_array* out, const hc_a
 int hc_fill_le(const hc_array* a, double threshold, double value, void* stream);
 
 /* ---- 3-D convolutional surrogate of the Frangi stack (no counterpart in the
- * reference; it stands in for filters/ridges.py:322-476 and is judged
- * against the plain kernels).  Valid 3x3x3 convolutions on channels-last
+ * reference; it stands in for filters/ridges.py:322-476 and its output is
+ * compared with the plain kernels').  Valid 3x3x3 convolutions on channels-last
  * bfloat16 activations, bf16 weights, float32 accumulation on the matrix
  * cores.  act: 0 none, 1 GELU (tanh form), 2 clamp to [0, 1]. */
 
