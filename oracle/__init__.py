@@ -2,8 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY.  Nothing in the ``cucim_amd`` package imports this
 package; only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline``
-leg of ``bench.py`` use it, and only as the checker (or, for the CPU baseline,
-as the restated CPU path that is timed next to the GPU).
+leg of ``bench.py`` use it, and only to check GPU results (or, for the CPU
+baseline, as the restated CPU path that is timed next to the GPU).
 
 The oracle restates the reference algorithms in numpy (float64 or an exact
 float32-FMA emulation of the reference's float32 accumulation).  Every function
