@@ -187,7 +187,42 @@ def bench_correlate(sizes=(2048, 8192)):
             "GBps_kernels": algo_bytes / (kern_ms * 1e-3) / 1e9,
             "wall_ms_per_call": ms_wall,
             "GBps_wall": algo_bytes / (ms_wall * 1e-3) / 1e9}
+
+        # skimage sobel magnitude (reflect), one fused launch: 4 B in + 4 B out
+        def run_s(i):
+            k = i % nbuf
+            filters.sobel(ins[k], mode="reflect")
+
+        for i in range(nbuf):
+            run_s(i)
+        with _lib.KernelTimer() as kt:
+            ms_wall = _events_time(run_s, 8 * nbuf)
+        kern_ms = sum(m for _, m in kt.records) / (8 * nbuf)
+        res[f"sobel2d_{n}"] = {
+            "kernels": sorted(set(k for k, _ in kt.records)),
+            "kernel_ms_per_call": kern_ms,
+            "GBps_kernels": algo_bytes / (kern_ms * 1e-3) / 1e9,
+            "wall_ms_per_call": ms_wall}
         del ins, outs, hin, hout
+        # rgb2gray of uint8 RGB: 3 B in + 4 B out per pixel
+        rgbs = [torch.randint(0, 256, (n, n, 3), generator=g, device="cuda",
+                              dtype=torch.uint8) for _ in range(max(2, nbuf * 8 // 7))]
+        from cucim_amd.skimage import color
+
+        def run_c(i):
+            color.rgb2gray(rgbs[i % len(rgbs)])
+
+        for i in range(len(rgbs)):
+            run_c(i)
+        with _lib.KernelTimer() as kt:
+            ms_wall = _events_time(run_c, 8 * len(rgbs))
+        kern_ms = sum(m for _, m in kt.records) / (8 * len(rgbs))
+        res[f"rgb2gray_{n}"] = {
+            "kernels": sorted(set(k for k, _ in kt.records)),
+            "kernel_ms_per_call": kern_ms,
+            "GBps_kernels": n * n * 7 / (kern_ms * 1e-3) / 1e9,
+            "wall_ms_per_call": ms_wall}
+        del rgbs
     return res
 
 

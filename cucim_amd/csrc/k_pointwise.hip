@@ -78,6 +78,52 @@ __global__ __launch_bounds__(256) void k_rgb2gray(const void* __restrict__ in, i
   }
 }
 
+// uint8 RGB -> float32 gray, 1024 pixels per wave and step: the wave loads
+// its 3072 interleaved bytes with three coalesced 16-byte loads per lane into
+// its own LDS slice, then each lane converts 4 consecutive pixels (three
+// aligned words) per round and writes them as one 16-byte store, so loads
+// and stores both cover contiguous spans.  The per-pixel arithmetic is that
+// of k_rgb2gray<float> (x * (float)(1/255) in float, then the double sum).
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4v_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_rgb2gray_u8(const u32x4_t* __restrict__ in,
+                                                     f32x4v_t* __restrict__ out, int64_t nwork,
+                                                     float scale) {
+  __shared__ __align__(16) unsigned int S[4][768];  // 3072 bytes per wave
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  u32x4_t* Sv = reinterpret_cast<u32x4_t*>(S[wv]);
+  for (int64_t w = (int64_t)blockIdx.x * 4 + wv; w < nwork; w += (int64_t)gridDim.x * 4) {
+    const u32x4_t* src = in + 192 * w;  // 1024 pixels = 192 16-byte chunks
+#pragma unroll
+    for (int k = 0; k < 3; ++k) Sv[lane + 64 * k] = src[lane + 64 * k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int j = lane + 64 * k;  // group of pixels 4 j .. 4 j + 3
+      const unsigned int wd[3] = {S[wv][3 * j], S[wv][3 * j + 1], S[wv][3 * j + 2]};
+      float g[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float c[3];
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+          const int byte = 3 * q + ch;
+          c[ch] = (float)((wd[byte >> 2] >> (8 * (byte & 3))) & 255u) * scale;
+        }
+        const double y = add_rn(add_rn(mul_rn(0.2125, (double)c[0]),
+                                       mul_rn(0.7154, (double)c[1])),
+                                mul_rn(0.0721, (double)c[2]));
+        g[q] = (float)y;
+      }
+      __builtin_nontemporal_store(f32x4v_t{g[0], g[1], g[2], g[3]}, out + 256 * w + j);
+    }
+    __builtin_amdgcn_wave_barrier();  // the slice is rewritten by the next step
+  }
+}
+
 template <typename C>
 __global__ __launch_bounds__(256) void k_to_float(const void* __restrict__ in, int dt,
                                                   void* __restrict__ out, int out_dt, int64_t n,
@@ -372,7 +418,20 @@ int hc_rgb2gray(const hc_array* in, const hc_array* out, void* stream) {
     HC_LAUNCH("k_rgb2gray", st, k_rgb2gray<double>, dim3((unsigned)grid_for(n)), dim3(256), 0, in->data, dt, out->data, out->dtype, n, sc, sgn);
   } else {
     const float sc = is_int(dt) ? (float)(1.0 / imax_of(dt)) : 0.0f;
-    HC_LAUNCH("k_rgb2gray", st, k_rgb2gray<float>, dim3((unsigned)grid_for(n)), dim3(256), 0, in->data, dt, out->data, out->dtype, n, sc, sgn);
+    int64_t done = 0;
+    if (dt == HC_UINT8 && out->dtype == HC_FLOAT32 &&
+        (((uintptr_t)in->data | (uintptr_t)out->data) & 15) == 0 && n >= 1024) {
+      // runs of 1024 pixels, 16-byte aligned in both arrays
+      const int64_t nw = n / 1024;
+      const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((nw + 3) / 4, 256 * 16));
+      HC_LAUNCH("k_rgb2gray_u8", st, k_rgb2gray_u8, dim3((unsigned)blocks), dim3(256), 0,
+                (const u32x4_t*)in->data, (f32x4v_t*)out->data, nw, sc);
+      done = nw * 1024;
+    }
+    if (done < n)
+      HC_LAUNCH("k_rgb2gray", st, k_rgb2gray<float>, dim3((unsigned)grid_for(n - done)), dim3(256),
+                0, (const uint8_t*)in->data + 3 * done * dtype_size(dt), dt,
+                (float*)out->data + done, out->dtype, n - done, sc, sgn);
   }
   HC_HIP(hipGetLastError());
   return HC_OK;
