@@ -142,6 +142,9 @@ int hc_tune_set(int knob, int value) {
   if (knob == HC_TUNE_SEP2D_TY && value != 0 && value != 8 && value != 16 && value != 32 &&
       value != 64)
     return hc::fail(HC_ERR_VALUE, "tile rows must be 8, 16, 32 or 64");
+  if (knob == HC_TUNE_EDGE_ROWS && value != 0 && value != 8 && value != 16 && value != 32 &&
+      value != 64 && value != 128)
+    return hc::fail(HC_ERR_VALUE, "edge rows per wave must be 8, 16, 32, 64 or 128");
   hc::g_tune[knob].store(value);
   return HC_OK;
 }

@@ -227,6 +227,247 @@ __global__ __launch_bounds__(256) void k_edge2d(const EdgeArgs<F> a) {
   }
 }
 
+// The same magnitude for a float input of the output's type, as a window
+// sliding down columns.  A wave owns a strip of 64 * VN columns (VN = 16 /
+// sizeof(F) consecutive columns per lane, one 16-byte load per row) and a run
+// of `rows` output rows; it reads rows y0 - 1 .. y0 + rows.  The columns left
+// and right of a lane's VN come from the neighbouring lanes (cross-lane
+// permute); the strip's two outer columns are loaded by lanes 0 and 63.  Rows
+// are fetched kEdgeU at a time, the next group's loads issued before the
+// current group is used.  Each output takes the taps in the same order as
+// k_edge2d, so the two kernels agree bit for bit.  Strips whose columns and
+// neighbours lie inside the image (and whose rows are 16-byte aligned) use
+// vector loads and streaming vector stores; the first and last strip, or
+// every strip when the width is not a multiple of VN, map each column
+// through the boundary mode and store per element.
+constexpr int kEdgeU = 4;
+
+template <typename F>
+struct EdgeVecT;
+template <>
+struct EdgeVecT<float> {
+  typedef float T __attribute__((ext_vector_type(4)));
+  static constexpr int N = 4;
+};
+template <>
+struct EdgeVecT<double> {
+  typedef double T __attribute__((ext_vector_type(2)));
+  static constexpr int N = 2;
+};
+
+template <typename F>
+struct EdgeColArgs {
+  const F* in;
+  F* out;
+  int64_t H, Wd;
+  int mode;
+  int nstrip, rows;
+  int64_t nwork;  // waves: nstrip * ceil(H / rows)
+  int vec;        // width a multiple of VN and both arrays 16-byte aligned
+  F e[3], s[3];
+};
+
+// Three taps in order.  Z >= 0: bit k set means weight k is zero, known at
+// compile time (the edge weights [w, 0, -w] of Sobel, Scharr and Prewitt with
+// nonzero smoothing weights); Z < 0: zero weights are tested at run time.
+template <int Z, typename F>
+__device__ __forceinline__ F tap3(F a, F b, F c, const F* w) {
+  F v = (F)0;
+  if (Z < 0) {
+    v = tap(v, a, w[0]);
+    v = tap(v, b, w[1]);
+    return tap(v, c, w[2]);
+  }
+  if (!(Z & 1)) v = fma_(a, w[0], v);
+  if (!(Z & 2)) v = fma_(b, w[1], v);
+  if (!(Z & 4)) v = fma_(c, w[2], v);
+  return v;
+}
+
+// remap() for positions at most one step outside the axis (p in [-1, n]),
+// as selects, without the general path's divisions.
+__device__ __forceinline__ int64_t remap_near(int64_t p, int64_t n, int mode) {
+  const bool lo = p < 0, hi = p >= n;
+  const bool mir = mode == HC_MODE_MIRROR, wrp = mode == HC_MODE_WRAP;
+  const int64_t at_lo = wrp ? n - 1 : (mir && n > 1 ? 1 : 0);
+  const int64_t at_hi = wrp ? 0 : (mir ? (n > 1 ? n - 2 : 0) : n - 1);
+  const int64_t r = lo ? at_lo : (hi ? at_hi : p);
+  return mode == HC_MODE_CONSTANT && (lo || hi) ? -1 : r;
+}
+
+// One row's loads, issued without branches or lane masks so that several
+// rows stay in flight: the VN columns of the lane and, in raw[VN], the
+// strip's outer column (lanes 0..31 the left one, 32..63 the right one, so
+// the load touches two cache lines).  Rows that read as zero (constant mode
+// outside the image, rows past y = H that feed no stored output) load row 0
+// and are zeroed where they are used.
+template <typename F, bool VEC>
+__device__ __forceinline__ bool edge_fetch(const EdgeColArgs<F>& a, int64_t y, int64_t x0,
+                                           int64_t xl, int lane, const int* cidx, F* raw) {
+  constexpr int VN = EdgeVecT<F>::N;
+  const int64_t yy = remap_near(y, a.H, a.mode);
+  const bool zero = y > a.H || yy < 0;
+  const F* row = a.in + (zero ? 0 : yy) * a.Wd;
+  if (VEC) {
+    const typename EdgeVecT<F>::T v = *reinterpret_cast<const typename EdgeVecT<F>::T*>(row + xl);
+#pragma unroll
+    for (int k = 0; k < VN; ++k) raw[k] = v[k];
+    raw[VN] = row[lane < 32 ? x0 - 1 : x0 + 64 * VN];
+  } else {
+#pragma unroll
+    for (int k = 0; k <= VN; ++k) raw[k] = row[cidx[k] < 0 ? 0 : cidx[k]];
+  }
+  return zero;
+}
+
+template <typename F, bool VEC, int ZE, int ZS>
+__device__ __forceinline__ void edge_strip(const EdgeColArgs<F>& a, int64_t y0, int64_t x0,
+                                           int lane) {
+  constexpr int VN = EdgeVecT<F>::N;
+  constexpr int U = VEC ? kEdgeU : 2;  // rows per fetch group; divides rows
+  const int64_t xl = x0 + (int64_t)lane * VN;
+  const bool cst = a.mode == HC_MODE_CONSTANT;
+  // Border strips: column indices mapped once; -1 reads as zero (constant
+  // mode outside the image, or a column past x = Wd that no stored output
+  // reads).  cidx[VN] is the strip's outer column for lanes 0 and 63.
+  int cidx[VN + 1];
+#pragma unroll
+  for (int k = 0; k <= VN; ++k) cidx[k] = 0;
+  if (!VEC) {
+#pragma unroll
+    for (int k = 0; k < VN; ++k) {
+      const int64_t x = xl + k;
+      cidx[k] = x <= a.Wd ? (int)remap_near(x, a.Wd, a.mode) : -1;
+    }
+    const int64_t xe = lane == 0 ? x0 - 1 : x0 + 64 * VN;
+    cidx[VN] = (lane == 0 || lane == 63) && xe <= a.Wd ? (int)remap_near(xe, a.Wd, a.mode) : -1;
+  }
+  F tm[VN + 2], t0[VN + 2], exm[VN], ex0[VN];
+#pragma unroll
+  for (int k = 0; k < VN + 2; ++k) tm[k] = t0[k] = (F)0;
+#pragma unroll
+  for (int k = 0; k < VN; ++k) exm[k] = ex0[k] = (F)0;
+
+  // row y's values are in (raw, zero); output row y - 1 when `out`
+  auto step = [&](const F* raw, bool zero, int64_t y, bool out) {
+    F c[VN + 1];
+#pragma unroll
+    for (int k = 0; k <= VN; ++k) {
+      bool z = zero;
+      if (!VEC) z = z || cidx[k] < 0;
+      c[k] = z ? (F)0 : raw[k];
+    }
+    // columns xl - 1 .. xl + VN of row y
+    F tp[VN + 2];
+#pragma unroll
+    for (int k = 0; k < VN; ++k) tp[k + 1] = c[k];
+    const F lft = __shfl_up(c[VN - 1], 1);
+    const F rgt = __shfl_down(c[0], 1);
+    tp[0] = lane == 0 ? c[VN] : lft;
+    tp[VN + 1] = lane == 63 ? c[VN] : rgt;
+    // edge along axis 1 of row y (zero outside the image in constant mode)
+    F exp_[VN];
+    const bool yout = cst && (y < 0 || y >= a.H);
+#pragma unroll
+    for (int k = 0; k < VN; ++k)
+      exp_[k] = yout ? (F)0 : tap3<ZE>(tp[k], tp[k + 1], tp[k + 2], a.e);
+    if (out) {
+      const int64_t yo = y - 1;
+      F ey[VN + 2];
+#pragma unroll
+      for (int j = 0; j < VN + 2; ++j) {
+        F v = tap3<ZE>(tm[j], t0[j], tp[j], a.e);
+        if (!VEC && cst) {
+          const int64_t x = xl - 1 + j;
+          if (x < 0 || x >= a.Wd) v = (F)0;
+        }
+        ey[j] = v;
+      }
+      F res[VN];
+#pragma unroll
+      for (int k = 0; k < VN; ++k) {
+        const F A = tap3<ZS>(ey[k], ey[k + 1], ey[k + 2], a.s);
+        const F B = tap3<ZS>(exm[k], ex0[k], exp_[k], a.s);
+        res[k] = sqrt(add_rn(mul_rn(A, A), mul_rn(B, B)));
+      }
+      if (yo < a.H) {
+        F* orow = a.out + yo * a.Wd;
+        if (VEC) {
+          typename EdgeVecT<F>::T o;
+#pragma unroll
+          for (int k = 0; k < VN; ++k) o[k] = res[k];
+          __builtin_nontemporal_store(o, reinterpret_cast<typename EdgeVecT<F>::T*>(orow + xl));
+        } else {
+#pragma unroll
+          for (int k = 0; k < VN; ++k)
+            if (xl + k < a.Wd) orow[xl + k] = res[k];
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < VN + 2; ++k) {
+      tm[k] = t0[k];
+      t0[k] = tp[k];
+    }
+#pragma unroll
+    for (int k = 0; k < VN; ++k) {
+      exm[k] = ex0[k];
+      ex0[k] = exp_[k];
+    }
+  };
+
+  // Rows y0 - 1 and y0 start the window; groups of U rows y0 + 1 .. y0 + rows
+  // follow, each group's loads issued before the previous group is used.  Two
+  // register sets alternate (no copies between them, which would make the
+  // compiler wait for loads in flight), and the fetches are unconditional
+  // inside the loop (a conditional fetch makes it wait at the join).
+  F pre[2][VN + 1], ga[U][VN + 1], gb[U][VN + 1];
+  bool zpre[2], za[U], zb[U];
+  auto fetch = [&](int g, F (*buf)[VN + 1], bool* z) {
+#pragma unroll
+    for (int i = 0; i < U; ++i)
+      z[i] = edge_fetch<F, VEC>(a, y0 + 1 + g * U + i, x0, xl, lane, cidx, buf[i]);
+  };
+  auto run = [&](int g, F (*buf)[VN + 1], const bool* z) {
+#pragma unroll
+    for (int i = 0; i < U; ++i) step(buf[i], z[i], y0 + 1 + g * U + i, true);
+  };
+#pragma unroll
+  for (int i = 0; i < 2; ++i) zpre[i] = edge_fetch<F, VEC>(a, y0 - 1 + i, x0, xl, lane, cidx, pre[i]);
+  fetch(0, ga, za);
+  step(pre[0], zpre[0], y0 - 1, false);
+  step(pre[1], zpre[1], y0, false);
+  const int nb = a.rows / U;  // at least 2
+  int g = 0;
+  for (; g + 2 < nb; g += 2) {
+    fetch(g + 1, gb, zb);
+    run(g, ga, za);
+    fetch(g + 2, ga, za);
+    run(g + 1, gb, zb);
+  }
+  if (g + 1 < nb) {
+    fetch(g + 1, gb, zb);
+    run(g, ga, za);
+    run(g + 1, gb, zb);
+  } else {
+    run(g, ga, za);
+  }
+}
+
+template <typename F, int ZE, int ZS>
+__global__ __launch_bounds__(256) void k_edge2d_cols(const EdgeColArgs<F> a) {
+  constexpr int VN = EdgeVecT<F>::N;
+  const int64_t g = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (g >= a.nwork) return;  // whole waves; the kernel has no barrier
+  const int lane = threadIdx.x & 63;
+  const int64_t chunk = g / a.nstrip, strip = g - chunk * a.nstrip;
+  const int64_t x0 = strip * 64 * VN, y0 = chunk * a.rows;
+  if (a.vec && x0 >= 1 && x0 + 64 * VN < a.Wd)
+    edge_strip<F, true, ZE, ZS>(a, y0, x0, lane);
+  else
+    edge_strip<F, false, ZE, ZS>(a, y0, x0, lane);
+}
+
 // ------------------------------------------------------ mask erosion --
 template <typename F>
 __global__ __launch_bounds__(256) void k_mask_eroded(F* __restrict__ out,
@@ -388,6 +629,48 @@ __global__ __launch_bounds__(256) void k_copy_vec(const u32x4* __restrict__ in,
   }
 }
 
+// Rows per wave: 8 unless the HC_TUNE_EDGE_ROWS knob says otherwise.  Short
+// runs measure fastest at 2048^2 and 8192^2 alike (the more waves, the more
+// rows in flight; the halo rows are mostly served by L2).
+template <typename F>
+int launch_edge_cols(const hc_array* in, const hc_array* out, const double* edge3,
+                     const double* smooth3, int mode, hipStream_t st) {
+  constexpr int VN = EdgeVecT<F>::N;
+  EdgeColArgs<F> a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = (const F*)in->data;
+  a.out = (F*)out->data;
+  a.H = in->shape[0];
+  a.Wd = in->shape[1];
+  a.mode = mode;
+  const int64_t nstrip = (a.Wd + 64 * VN - 1) / (64 * VN);
+  int rows = tune_get(HC_TUNE_EDGE_ROWS);
+  if (rows == 0) rows = 8;
+  const int64_t nwork = nstrip * ((a.H + rows - 1) / rows);
+  if (a.Wd >= INT32_MAX || (nwork + 3) / 4 > INT32_MAX)
+    return fail(HC_ERR_VALUE, "image too large for the edge kernel's grid");
+  a.nstrip = (int)nstrip;
+  a.rows = rows;
+  a.nwork = nwork;
+  a.vec = (a.Wd % VN == 0) && ((uintptr_t)a.in % 16 == 0) && ((uintptr_t)a.out % 16 == 0);
+  for (int k = 0; k < 3; ++k) {
+    a.e[k] = (F)edge3[k];
+    a.s[k] = (F)smooth3[k];
+  }
+  int ze = 0, zs = 0;
+  for (int k = 0; k < 3; ++k) {
+    ze |= a.e[k] == (F)0 ? 1 << k : 0;
+    zs |= a.s[k] == (F)0 ? 1 << k : 0;
+  }
+  const dim3 grid((unsigned)((nwork + 3) / 4));
+  if (ze == 2 && zs == 0)
+    HC_LAUNCH("k_edge2d_cols", st, (k_edge2d_cols<F, 2, 0>), grid, dim3(256), 0, a);
+  else
+    HC_LAUNCH("k_edge2d_cols", st, (k_edge2d_cols<F, -1, -1>), grid, dim3(256), 0, a);
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
 }  // namespace hc
 
 using namespace hc;
@@ -503,6 +786,11 @@ int hc_edge_magnitude(const hc_array* in, const hc_array* out, const double* edg
   if (H == 0 || Wd == 0) return HC_OK;
   hipStream_t st = (hipStream_t)stream;
   const int dt = in->dtype;
+  if (dt == out->dtype) {
+    if (dt == HC_FLOAT32)
+      return launch_edge_cols<float>(in, out, edge3, smooth3, mode, st);
+    return launch_edge_cols<double>(in, out, edge3, smooth3, mode, st);
+  }
   const int ntx = (int)((Wd + kEdgeTX - 1) / kEdgeTX), nty = (int)((H + kEdgeTY - 1) / kEdgeTY);
   const int64_t ntiles = (int64_t)ntx * nty;
   if (out->dtype == HC_FLOAT64) {

@@ -112,6 +112,9 @@ typedef enum hc_tune_knob {
                                only, tap by tap as the reference computes
                                (default 0: float32 passes on the matrix
                                cores where they apply) */
+  HC_TUNE_EDGE_ROWS = 3,    /* fused 2-D edge magnitude (float input of the
+                               output's type): output rows per wave, 8, 16,
+                               32, 64 or 128 */
   HC_TUNE_NKNOBS = 8
 } hc_tune_knob;
 int hc_tune_set(int knob, int value);

@@ -106,6 +106,64 @@ def test_sobel_vs_oracle(gpu, dtype, mode, shape, axis):
     np.testing.assert_allclose(got, exp, rtol=tol, atol=tol)
 
 
+def _edge_magnitude_emulated(x, smooth, mode):
+    """float32 magnitude with the reference's own arithmetic
+    (``filters/edges.py:218-241``): each 1-D pass an FMA chain over the taps
+    in order (the oracle's exact float32 emulation), results stored in
+    float32, then ``ax * ax``, ``+`` and ``sqrt``, each rounded to float32."""
+    edge = (np.asarray([1, 0, -1], np.float32) / math.sqrt(2)).astype(np.float32)
+    sm = np.asarray(smooth, np.float32)
+    out = None
+    for e_ax, s_ax in ((0, 1), (1, 0)):
+        a = on.convolve1d(x, edge, axis=e_ax, output=np.float32, mode=mode, acc="emulate")
+        a = on.convolve1d(a, sm, axis=s_ax, output=np.float32, mode=mode, acc="emulate")
+        a = a * a
+        out = a if out is None else out + a
+    return np.sqrt(out)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 3), (3, 1100), (37, 64), (70, 1028),
+                                   (129, 1024), (33, 1030), (300, 515), (17, 2049)])
+@pytest.mark.parametrize("mode", ["reflect", "nearest", "constant", "wrap", "mirror"])
+@pytest.mark.parametrize("rows", [0, 32, 128])
+def test_edge_magnitude_bit_exact(gpu, shape, mode, rows):
+    """The fused float32 magnitude (column-sliding kernel: vector strips inside
+    the image, per-element strips at the borders and for widths that are not
+    a multiple of 4) equals the reference's pass-by-pass float32 arithmetic
+    bit for bit, for each rows-per-wave setting."""
+    from cucim_amd import _lib
+    from cucim_amd.skimage.filters import prewitt, sobel
+
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal(shape).astype(np.float32)
+    lib = _lib.lib()
+    _lib.check(lib.hc_tune_set(_lib.TUNE_EDGE_ROWS, rows))
+    try:
+        for fn, smooth in ((sobel, np.array([1, 2, 1]) / 4), (prewitt, np.full(3, 1 / 3))):
+            got = _np(fn(_t(x), mode=mode))
+            exp = _edge_magnitude_emulated(x, smooth, mode)
+            np.testing.assert_array_equal(got.view(np.uint32), exp.view(np.uint32))
+    finally:
+        lib.hc_tune_set(_lib.TUNE_EDGE_ROWS, 0)
+
+
+def test_edge_magnitude_unaligned_and_knob(gpu):
+    """A view that starts 4 bytes into its buffer takes the per-element
+    strips; the rows knob accepts only its listed values."""
+    from cucim_amd import _lib
+    from cucim_amd.skimage.filters import sobel
+
+    rng = np.random.default_rng(22)
+    base = _t(rng.standard_normal(40 * 1200 + 1).astype(np.float32))
+    x = base[1:].view(40, 1200)
+    got = _np(sobel(x))
+    exp = _edge_magnitude_emulated(_np(x), np.array([1, 2, 1]) / 4, "reflect")
+    np.testing.assert_array_equal(got.view(np.uint32), exp.view(np.uint32))
+    lib = _lib.lib()
+    assert lib.hc_tune_set(_lib.TUNE_EDGE_ROWS, 12) == _lib.ERR_VALUE
+    assert lib.hc_tune_get(_lib.TUNE_EDGE_ROWS) == 0
+
+
 def test_sobel_mask(gpu):
     from cucim_amd.skimage.filters import sobel
 
