@@ -178,7 +178,14 @@ struct GgsArgs {
   float e[3], s[3];  // edge / smoothing correlation weights
 };
 
-__device__ __forceinline__ float gray_u8(const uint8_t* __restrict__ p, float scale) {
+// Image pointers read from the tile table are generic; as generic (flat)
+// accesses every load would also count against the LDS counter, and the
+// compiler then waits for all loads in flight before using any.  They are
+// global memory, so they are cast to the global address space.
+typedef const __attribute__((address_space(1))) uint8_t* gu8_ptr;
+typedef __attribute__((address_space(1))) float* gf32_ptr;
+
+__device__ __forceinline__ float gray_u8(gu8_ptr p, float scale) {
   const double r = (double)((float)p[0] * scale);
   const double g = (double)((float)p[1] * scale);
   const double b = (double)((float)p[2] * scale);
@@ -210,7 +217,7 @@ __global__ __launch_bounds__(256) void k_tiles_ggs(const GgsArgs a, const RegTap
     const int64_t nbx = (W + a.TX - 1) / a.TX;
     const int64_t by = local / nbx, bx = local - by * nbx;
     const int64_t y0 = by * TY, x0 = bx * a.TX;
-    const uint8_t* in = reinterpret_cast<const uint8_t*>(d.in);
+    const gu8_ptr in = (gu8_ptr)d.in;
     // 1. gray + vertical gaussian pass, one staged column per lane
     {
       const int64_t xr = remap(x0 - 1 - a.ag + c, W, a.gmode);
@@ -219,7 +226,7 @@ __global__ __launch_bounds__(256) void k_tiles_ggs(const GgsArgs a, const RegTap
       if (ya >= 0 && ya + GR + K - 1 <= H) {
         // every staged row inside the tile: consecutive rows, no boundary
         // mapping
-        const uint8_t* col = in + 3 * (ya * W + (xr >= 0 ? xr : 0));
+        const gu8_ptr col = in + 3 * (ya * W + (xr >= 0 ? xr : 0));
 #pragma unroll
         for (int j = 0; j < GR + K - 1; ++j) v[j] = gray_u8(col + 3 * W * j, a.scale);
         if (xr < 0) {
@@ -282,7 +289,7 @@ __global__ __launch_bounds__(256) void k_tiles_ggs(const GgsArgs a, const RegTap
     }
     // 4. edge magnitude, one output column per lane
     if (c < a.TX && x0 + c < W) {
-      float* out = reinterpret_cast<float*>(d.out);
+      const gf32_ptr out = (gf32_ptr)d.out;
       // a 3x3 window slides down the lane's column: G rows i, i + 1, i + 2
       // (image rows y - 1 .. y + 1) at columns x - 1 .. x + 1, and the
       // horizontal edge response of each row, are carried to the next row
