@@ -94,11 +94,15 @@ __device__ __forceinline__ void eig33(F aa, F bb, F cc, F dd, F ee, F ff, int de
     phi = atan(sqrt(arg) / x2);
     if (x2 < 0) phi += M_PI;
   }
+  // the reference divides by 3; a product with the double nearest 1/3
+  // differs by at most an ulp of double, far below the float rounding that
+  // follows, and avoids four IEEE divisions
+  constexpr double third = 1.0 / 3.0;
   const double x1_term = (2.0 / 3.0) * sqrt(x1);
-  const double abc = (a + b + c) / 3.0;
-  F lam1 = (F)(abc - x1_term * cos_small(phi / 3.0));
-  F lam2 = (F)(abc + x1_term * cos_small((phi - M_PI) / 3.0));
-  F lam3 = (F)(abc + x1_term * cos_small((phi + M_PI) / 3.0));
+  const double abc = (a + b + c) * third;
+  F lam1 = (F)(abc - x1_term * cos_small(phi * third));
+  F lam2 = (F)(abc + x1_term * cos_small((phi - M_PI) * third));
+  F lam3 = (F)(abc + x1_term * cos_small((phi + M_PI) * third));
   // the network runs on (var1, lam2, var3) = (lam1, lam2, lam3) ascending,
   // (lam3, lam2, lam1) descending
   F& v1 = descending ? lam3 : lam1;
