@@ -160,6 +160,50 @@ __device__ __forceinline__ F frangi_v3(F l1, F l2, F l3, F r_g, double alpha_sq,
   return result;
 }
 
+// The fused passes' forms of frangi_v2 / frangi_v3: the same double
+// exponentials, with each argument -r / s_sq computed as r * (-1 / s_sq) and
+// the reciprocals taken once per kernel.  A product with the rounded
+// reciprocal differs from the quotient by at most an ulp of double, far below
+// the rounding to F that follows, and saves three double divisions per voxel.
+struct FrangiNegRecip {
+  double a, b, g;  // -1 / alpha_sq, -1 / beta_sq, -1 / gamma_sq
+};
+// a wave-uniform double kept in scalar registers (the divisions run on the
+// vector ALU, which would otherwise hold the results in vector registers)
+__device__ __forceinline__ double uniform_d(double x) {
+  const unsigned long long u = __double_as_longlong(x);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// alpha_sq, beta_sq, gamma_sq must be the same in every lane of a wave
+__device__ __forceinline__ FrangiNegRecip frangi_neg_recip(double alpha_sq, double beta_sq,
+                                                           double gamma_sq) {
+  return {uniform_d(-1.0 / alpha_sq), uniform_d(-1.0 / beta_sq), uniform_d(-1.0 / gamma_sq)};
+}
+
+template <typename F>
+__device__ __forceinline__ F frangi_v2r(F l1, F l2, F r_g, const FrangiNegRecip& q) {
+  F r_b = fabs(l1) / fmax(l2, (F)1.0e-10);
+  r_b *= r_b;
+  F result = (F)exp((double)r_b * q.b);
+  result = (F)((double)result * (1.0 - exp((double)r_g * q.g)));
+  return result;
+}
+
+template <typename F>
+__device__ __forceinline__ F frangi_v3r(F l1, F l2, F l3, F r_g, const FrangiNegRecip& q) {
+  const F lam2 = fmax(l2, (F)1.0e-10);
+  const F lam3 = fmax(l3, (F)1.0e-10);
+  F r_a = lam2 / lam3;
+  r_a *= r_a;
+  const F r_b = (l1 * l1) / (lam2 * lam3);
+  F result = (F)(1.0 - exp((double)r_a * q.a));
+  result = (F)((double)result * exp((double)r_b * q.b));
+  result = (F)((double)result * (1.0 - exp((double)r_g * q.g)));
+  return result;
+}
+
 // cp.maximum: NaN-propagating
 template <typename F>
 __device__ __forceinline__ F nanmax(F a, F b) {

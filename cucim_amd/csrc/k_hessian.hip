@@ -799,6 +799,7 @@ __global__ __launch_bounds__(256) void k_final_contig(const FinalArgs<W> a) {
   // epilogue: every item holds all Hessian elements of its P voxels
   double gsq = a.gamma_sq;
   if (a.epi == EPI_FRANGI && a.gamma_sq_dev) gsq = *a.gamma_sq_dev;
+  const FrangiNegRecip nrc = frangi_neg_recip(a.alpha_sq, a.beta_sq, gsq);
   W rgmax = (W)0;
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
@@ -828,7 +829,7 @@ __global__ __launch_bounds__(256) void k_final_contig(const FinalArgs<W> a) {
           a.eig[2][idx] = l3;
           rgmax = nanmax<W>(rgmax, rg);
         } else {
-          const W v = frangi_v3<W>(l1, l2, l3, rg, a.alpha_sq, a.beta_sq, gsq);
+          const W v = frangi_v3r<W>(l1, l2, l3, rg, nrc);
           a.fout[idx] = a.first_sigma ? v : nanmax<W>(a.fout[idx], v);
         }
       } else {
@@ -840,7 +841,7 @@ __global__ __launch_bounds__(256) void k_final_contig(const FinalArgs<W> a) {
           a.eig[1][idx] = l2;
           rgmax = nanmax<W>(rgmax, rg);
         } else {
-          const W v = frangi_v2<W>(l1, l2, rg, a.beta_sq, gsq);
+          const W v = frangi_v2r<W>(l1, l2, rg, nrc);
           a.fout[idx] = a.first_sigma ? v : nanmax<W>(a.fout[idx], v);
         }
       }
@@ -975,6 +976,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
   halo_table(hsrc, g.E, g.N, g.shift, a.mode, tid, NT);
   double gsq = a.gamma_sq;
   if (a.epi == EPI_FRANGI && a.gamma_sq_dev) gsq = *a.gamma_sq_dev;
+  const FrangiNegRecip nrc = frangi_neg_recip(a.alpha_sq, a.beta_sq, gsq);
   const int64_t N = g.N;
   float rgmax = 0.f;
   float pf[NPF];
@@ -1013,7 +1015,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
             a.eig[2][idx] = l3;
             rgmax = nanmax<float>(rgmax, rg);
           } else {
-            const float vv = frangi_v3<float>(l1, l2, l3, rg, a.alpha_sq, a.beta_sq, gsq);
+            const float vv = frangi_v3r<float>(l1, l2, l3, rg, nrc);
             a.fout[idx] = a.first_sigma ? vv : nanmax<float>(a.fout[idx], vv);
           }
         } else {
@@ -1025,7 +1027,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
             a.eig[1][idx] = l2;
             rgmax = nanmax<float>(rgmax, rg);
           } else {
-            const float vv = frangi_v2<float>(l1, l2, rg, a.beta_sq, gsq);
+            const float vv = frangi_v2r<float>(l1, l2, rg, nrc);
             a.fout[idx] = a.first_sigma ? vv : nanmax<float>(a.fout[idx], vv);
           }
         }
