@@ -949,7 +949,11 @@ __device__ __forceinline__ void final_all_mfma(const FinalArgs<float>& a, const 
    ...);
 }
 
-template <int NIN, int NBW, int NW>
+// EP: the epilogue, fixed at compile time so that each variant keeps only
+// its own arguments in scalar registers: 0 store the Hessian (EPI_STORE),
+// 1 Frangi storing eigenvalues and reducing max r_g (first sigma, gamma
+// unknown), 2 Frangi vesselness into the running maximum.
+template <int NIN, int NBW, int NW, int EP>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void k_final_mfma(
     const FinalArgs<float> a) {
   constexpr int NPF = kMfmaNpf, NT = 64 * NW;
@@ -975,7 +979,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
   stage_trimmed(wpS, wpD, a.S, a.D, a.t0, a.Km, WP, tid, NT);
   halo_table(hsrc, g.E, g.N, g.shift, a.mode, tid, NT);
   double gsq = a.gamma_sq;
-  if (a.epi == EPI_FRANGI && a.gamma_sq_dev) gsq = *a.gamma_sq_dev;
+  if (EP != 0 && a.gamma_sq_dev) gsq = *a.gamma_sq_dev;
   const FrangiNegRecip nrc = frangi_neg_recip(a.alpha_sq, a.beta_sq, gsq);
   const int64_t N = g.N;
   float rgmax = 0.f;
@@ -999,7 +1003,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
 #pragma unroll
         for (int j = 0; j < NIN; ++j) h[j] = res[j][i][v & 3];
         const int64_t idx = row * N + p;
-        if (a.epi == EPI_STORE) {
+        if (EP == 0) {
 #pragma unroll
           for (int j = 0; j < NIN; ++j) a.out[j][idx] = h[j];
           continue;
@@ -1009,7 +1013,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
           // H order (H00, H01, H02, H11, H12, H22) = (a, d, f, b, e, c)
           eig33<float>(h[0], h[3 % NIN], h[5 % NIN], h[1], h[4 % NIN], h[2], 0, 1, l1, l2, l3);
           const float rg = frangi_rg3<float>(l1, l2, l3);
-          if (a.store_eig) {
+          if (EP == 1) {
             a.eig[0][idx] = l1;
             a.eig[1][idx] = l2;
             a.eig[2][idx] = l3;
@@ -1022,7 +1026,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
           float l1, l2;
           eig22<float>(h[0], h[1], h[2], 0, 1, l1, l2);
           const float rg = frangi_rg2<float>(l1, l2);
-          if (a.store_eig) {
+          if (EP == 1) {
             a.eig[0][idx] = l1;
             a.eig[1][idx] = l2;
             rgmax = nanmax<float>(rgmax, rg);
@@ -1034,7 +1038,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
       }
     }
   }
-  if (a.epi == EPI_FRANGI && a.store_eig) atomic_max_bits<float>(a.rg_max, rgmax);
+  if (EP == 1) atomic_max_bits<float>(a.rg_max, rgmax);
 }
 
 // ---------------------------------------------------- small kernels --
@@ -1331,18 +1335,25 @@ int prepare_final(FinalArgs<W>* a, const hc_array* shape_src, const AxisTaps<W>&
   return HC_OK;
 }
 
-template <int NIN, int NBW, int NW>
-int launch_final_mfma_nbw(const FinalArgs<float>& a, const MfmaGeomHost& g, hipStream_t st) {
-  const void* f = (const void*)k_final_mfma<NIN, NBW, NW>;
+template <int NIN, int NBW, int NW, int EP>
+int launch_final_mfma_ep(const FinalArgs<float>& a, const MfmaGeomHost& g, hipStream_t st) {
+  const void* f = (const void*)k_final_mfma<NIN, NBW, NW, EP>;
   if (g.lds > 64 * 1024) {
     const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, g.lds);
     if (e != hipSuccess) return hip_check(e, "hipFuncSetAttribute(k_final_mfma)");
   }
   const int64_t G = (a.nrows + kChC - 1) / kChC;
-  HC_LAUNCH("k_final_mfma", st, (k_final_mfma<NIN, NBW, NW>), dim3((unsigned)G), dim3(64 * NW),
-            g.lds, a);
+  HC_LAUNCH("k_final_mfma", st, (k_final_mfma<NIN, NBW, NW, EP>), dim3((unsigned)G),
+            dim3(64 * NW), g.lds, a);
   HC_HIP(hipGetLastError());
   return HC_OK;
+}
+
+template <int NIN, int NBW, int NW>
+int launch_final_mfma_nbw(const FinalArgs<float>& a, const MfmaGeomHost& g, hipStream_t st) {
+  if (a.epi == EPI_STORE) return launch_final_mfma_ep<NIN, NBW, NW, 0>(a, g, st);
+  if (a.store_eig) return launch_final_mfma_ep<NIN, NBW, NW, 1>(a, g, st);
+  return launch_final_mfma_ep<NIN, NBW, NW, 2>(a, g, st);
 }
 
 // Matrix-core final pass for float32 rows of at most 512 voxels; returns -1
