@@ -424,10 +424,37 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(5))) vo
     const bool ok = L < nlines;
     const int64_t lo = ok ? L / inner : 0, li = ok ? L - lo * inner : 0;
     const int64_t base = lo * N * inner + li;
+    if (jb.in_dt == HC_FLOAT32) {
+      // All NPF loads are issued before any is waited for.  Written as plain
+      // code, the compiler turns the zeroing of positions and lines outside
+      // into a lane-masked load per element followed by a wait, so the loads
+      // of a lane would run one after another.  Here every load is
+      // unconditional and in bounds (index 0 stands in for what lies
+      // outside), issued by inline asm, followed by one wait for all of
+      // them; each value is tied to that wait before it is used.
+      const float* in = reinterpret_cast<const float*>(jb.in);
+      const bool neg = a.negate != 0;
 #pragma unroll
-    for (int u = 0; u < NPF; ++u) {
-      const int n = ns0 + 4 * NW * u;
-      v[u] = (ok && n < N) ? load_in(jb.in, jb.in_dt, base + n * inner, a.negate != 0) : 0.f;
+      for (int u = 0; u < NPF; ++u) {
+        const int n = ns0 + 4 * NW * u;
+        const bool use = ok && n < N;
+        const float* p = in + base + (use ? (int64_t)n : 0) * inner;
+        asm volatile("global_load_dword %0, %1, off" : "=v"(v[u]) : "v"(p) : "memory");
+      }
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+#pragma unroll
+      for (int u = 0; u < NPF; ++u) {
+        asm volatile("" : "+v"(v[u]));
+        const int n = ns0 + 4 * NW * u;
+        const bool use = ok && n < N;
+        v[u] = use ? (neg ? -v[u] : v[u]) : 0.f;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < NPF; ++u) {
+        const int n = ns0 + 4 * NW * u;
+        v[u] = (ok && n < N) ? load_in(jb.in, jb.in_dt, base + n * inner, a.negate != 0) : 0.f;
+      }
     }
   };
   const bool cval_bad = !finite(a.cval);
