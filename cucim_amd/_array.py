@@ -158,8 +158,9 @@ def get_output(output, inp, shape=None):
 
 
 def binary(a, b, out, op):
-    """out = a - b (``_lib.BIN_SUB``) or a * b (``_lib.BIN_MUL``), one rounding
-    per element in the arrays' float dtype; out may be a or b."""
+    """out = a - b (``_lib.BIN_SUB``), a * b (``_lib.BIN_MUL``) or a + b
+    (``_lib.BIN_ADD``): one rounding per element in the arrays' float dtype,
+    wrapping for integer dtypes; out may be a or b."""
     _lib.check(_lib.lib().hc_binary(hc_array(a), hc_array(b), hc_array(out), op, stream()),
                "elementwise op")
     return out

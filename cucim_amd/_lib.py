@@ -27,7 +27,7 @@ ALGO_AUTO, ALGO_TILED, ALGO_GATHER = range(3)
 
 # hc_tune_knob
 TUNE_CONTIG_SEG, TUNE_SEP2D_TY, TUNE_HESSIAN_VALU, TUNE_EDGE_ROWS = 0, 1, 2, 3
-BIN_SUB, BIN_MUL = 0, 1
+BIN_SUB, BIN_MUL, BIN_ADD = 0, 1, 2
 
 # hc_status
 OK, ERR_VALUE, ERR_RESOURCE, ERR_HIP, ERR_UNSUPPORTED = range(5)
@@ -56,6 +56,8 @@ ENTRY_POINTS = (
     "hc_maximum_inplace",
     "hc_square_acc",
     "hc_binary",
+    "hc_div_scalar",
+    "hc_correlate_sparse",
     "hc_sqrt_inplace",
     "hc_copy_cast",
     "hc_batch2d_plan",
@@ -123,8 +125,8 @@ _SIGNATURES = {
     "hc_separable2d": ([_A, _A, _T, _T, _i, _i, _d, _v], _i),
     "hc_hessian_gaussian": ([_A, _A, _i, _T, _T, _i, _d, _i, _v], _i),
     "hc_sym_eigvals": ([_A, _i, _A, _i, _i, _v], _i),
-    "hc_frangi": ([_A, _A, _T, _T, _i, _d, _d, _d, _i, _i, _d, _A, _i, _v, _v], _i),
-    "hc_frangi_from_eigvals": ([_A, _i, _A, _d, _d, _d, _i, _v, _v], _i),
+    "hc_frangi": ([_A, _A, _T, _T, _i, _d, _d, _d, _i, _i, _i, _d, _A, _i, _v, _v], _i),
+    "hc_frangi_from_eigvals": ([_A, _i, _A, _d, _d, _d, _i, _i, _v, _v], _i),
     "hc_rgb2gray": ([_A, _A, _v], _i),
     "hc_to_float": ([_A, _A, _v], _i),
     "hc_gradient": ([_A, _A, _i, _v], _i),
@@ -133,6 +135,8 @@ _SIGNATURES = {
     "hc_maximum_inplace": ([_A, _A, _v], _i),
     "hc_square_acc": ([_A, _A, _i, _v], _i),
     "hc_binary": ([_A, _A, _A, _i, _v], _i),
+    "hc_div_scalar": ([_A, _d, _v], _i),
+    "hc_correlate_sparse": ([_A, _A, _v, _v, _v, _i, _i, _i, _v], _i),
     "hc_sqrt_inplace": ([_A, _v], _i),
     "hc_copy_cast": ([_A, _A, _v], _i),
     "hc_batch2d_plan": ([_v, _i, _i, _i, _v, ctypes.c_int64], ctypes.c_int64),

@@ -339,4 +339,8 @@ bool same_shape(const hc_array* a, const hc_array* b);
 template <typename W>
 int make_taps(const hc_taps* t, int pad_to, TapPack<W>* out);
 
+// Eigenvalues of n x n symmetric matrices (n != 2, 3), k_eig_nd.hip.
+int sym_eigvals_nd(const hc_array* S, int nS, const hc_array* eig, int sort, int abs_sort,
+                   hipStream_t st);
+
 }  // namespace hc

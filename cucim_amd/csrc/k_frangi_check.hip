@@ -56,7 +56,18 @@ __global__ __launch_bounds__(256) void k_frangi_check(const CheckArgs a) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int x0 = a.lines[4 * b], y0 = a.lines[4 * b + 1];
   const int za = a.lines[4 * b + 2], zb = a.lines[4 * b + 3];
-  const int ns = zb - za;
+  // a line whose windows would leave the volume (or that is longer than the
+  // sample buffer) is not evaluated: ns = 0 marks it as without samples
+  int ns = zb - za;
+  if (ns < 0 || ns > kChkSamples) ns = 0;
+  for (int s = 0; s < a.nsig; ++s) {
+    const int L = a.kinfo[3 * s + 1], A = a.kinfo[3 * s + 2];
+    const bool inside = L >= 1 && L <= kChkMaxL && A >= 0 && za - A >= 0 &&
+                        (int64_t)za - A + ns + L - 1 <= a.D && y0 - A >= 0 &&
+                        (int64_t)y0 - A + L <= a.H && x0 - A >= 0 &&
+                        (int64_t)x0 - A + L <= a.W;
+    if (!inside) ns = 0;
+  }
   if (tid < kChkSamples) res[tid] = 0.f;
   __syncthreads();
   // element e: (z kernel, y kernel, x kernel) indices into {SS, SD, DD}

@@ -425,26 +425,22 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(5))) vo
     const int64_t lo = ok ? L / inner : 0, li = ok ? L - lo * inner : 0;
     const int64_t base = lo * N * inner + li;
     if (jb.in_dt == HC_FLOAT32) {
-      // All NPF loads are issued before any is waited for.  Written as plain
-      // code, the compiler turns the zeroing of positions and lines outside
-      // into a lane-masked load per element followed by a wait, so the loads
-      // of a lane would run one after another.  Here every load is
+      // All NPF loads are issued before any is waited for: every load is
       // unconditional and in bounds (index 0 stands in for what lies
-      // outside), issued by inline asm, followed by one wait for all of
-      // them; each value is tied to that wait before it is used.
+      // outside the line or past the last line), and the zeroing is applied
+      // to the loaded values afterwards.  Written as a conditional load, the
+      // compiler emits a lane-masked load per element followed by its wait,
+      // so the loads of a lane would run one after another.
       const float* in = reinterpret_cast<const float*>(jb.in);
       const bool neg = a.negate != 0;
 #pragma unroll
       for (int u = 0; u < NPF; ++u) {
         const int n = ns0 + 4 * NW * u;
         const bool use = ok && n < N;
-        const float* p = in + base + (use ? (int64_t)n : 0) * inner;
-        asm volatile("global_load_dword %0, %1, off" : "=v"(v[u]) : "v"(p) : "memory");
+        v[u] = in[base + (use ? (int64_t)n : 0) * inner];
       }
-      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 #pragma unroll
       for (int u = 0; u < NPF; ++u) {
-        asm volatile("" : "+v"(v[u]));
         const int n = ns0 + 4 * NW * u;
         const bool use = ok && n < N;
         v[u] = use ? (neg ? -v[u] : v[u]) : 0.f;
@@ -1532,7 +1528,8 @@ int hessian_impl(const hc_array* in, const hc_array* H, int nH, const hc_taps* t
 
 template <typename W>
 int frangi_impl(const hc_array* in, const hc_array* out, const hc_taps* ts, const hc_taps* td,
-                int nsig, double alpha, double beta, double gamma, int black_ridges, int mode,
+                int nsig, double alpha, double beta, double gamma, bool gamma_known,
+                int black_ridges, int mode,
                 double cval, const hc_array* work, int nwork, void* scratch, hipStream_t st) {
   const int nd = in->ndim;
   const int nH = nd == 3 ? 6 : 3;
@@ -1558,7 +1555,6 @@ int frangi_impl(const hc_array* in, const hc_array* out, const hc_taps* ts, cons
   unsigned long long* rg = reinterpret_cast<unsigned long long*>(scratch);
   double* gsq_dev = reinterpret_cast<double*>(reinterpret_cast<char*>(scratch) + 16);
   const double alpha_sq = 2 * alpha * alpha, beta_sq = 2 * beta * beta;
-  const bool gamma_known = gamma >= 0;
   const double gamma_sq = gamma_known ? 2 * gamma * gamma : 0.0;
   Pair P[6];
   hessian_pairs(nd, 0, P);
@@ -1611,11 +1607,11 @@ int frangi_impl(const hc_array* in, const hc_array* out, const hc_taps* ts, cons
 
 template <typename W>
 int frangi_eig_impl(const hc_array* eig, int nd, const hc_array* out, double alpha, double beta,
-                    double gamma, int first_sigma, void* scratch, hipStream_t st) {
+                    double gamma, bool known, int first_sigma, void* scratch,
+                    hipStream_t st) {
   const int64_t n = numel(out);
   if (n == 0) return HC_OK;
   const double alpha_sq = 2 * alpha * alpha, beta_sq = 2 * beta * beta;
-  const bool known = gamma >= 0;
   const W* e0 = (const W*)eig[0].data;
   const W* e1 = (const W*)eig[1].data;
   const W* e2 = (const W*)eig[nd == 3 ? 2 : 1].data;
@@ -1707,8 +1703,10 @@ int hc_hessian_gaussian(const hc_array* in, const hc_array* H, int nH, const hc_
 int hc_sym_eigvals(const hc_array* S, int nS, const hc_array* eig, int sort, int abs_sort,
                    void* stream) {
   if (!S || !eig) return fail(HC_ERR_VALUE, "null argument");
-  if (nS != 3 && nS != 6) return fail(HC_ERR_VALUE, "closed-form eigenvalues need 3 or 6 elements");
-  const int ne = nS == 6 ? 3 : 2;
+  int ne = 0;  // matrix order: nS = ne (ne + 1) / 2
+  while (ne < 8 && ne * (ne + 1) / 2 < nS) ++ne;
+  if (nS < 1 || ne * (ne + 1) / 2 != nS)
+    return fail(HC_ERR_VALUE, "element count must be n(n+1)/2 for a matrix order 1..8");
   const int dt = S[0].dtype;
   if (dt != HC_FLOAT32 && dt != HC_FLOAT64)
     return fail(HC_ERR_VALUE, "expected real-valued floating point matrices");
@@ -1721,6 +1719,8 @@ int hc_sym_eigvals(const hc_array* S, int nS, const hc_array* eig, int sort, int
   const int64_t n = numel(&S[0]);
   if (n == 0) return HC_OK;
   hipStream_t st = (hipStream_t)stream;
+  // other orders: the iterative solver of the n-dimensional branch
+  if (nS != 3 && nS != 6) return sym_eigvals_nd(S, nS, eig, sort, abs_sort, st);
   const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 64);
   const void* s[6];
   for (int k = 0; k < 6; ++k) s[k] = S[k < nS ? k : 0].data;
@@ -1735,8 +1735,8 @@ int hc_sym_eigvals(const hc_array* S, int nS, const hc_array* eig, int sort, int
 }
 
 int hc_frangi_from_eigvals(const hc_array* eig, int ndim, const hc_array* out, double alpha,
-                           double beta, double gamma, int first_sigma, void* scratch,
-                           void* stream) {
+                           double beta, double gamma, int has_gamma, int first_sigma,
+                           void* scratch, void* stream) {
   if (!eig || !out || !scratch) return fail(HC_ERR_VALUE, "null argument");
   if (ndim != 2 && ndim != 3) return fail(HC_ERR_VALUE, "frangi handles 2-D and 3-D images");
   if (out->dtype != HC_FLOAT32 && out->dtype != HC_FLOAT64)
@@ -1747,14 +1747,16 @@ int hc_frangi_from_eigvals(const hc_array* eig, int ndim, const hc_array* out, d
       return fail(HC_ERR_VALUE, "eigenvalue arrays must match the output");
   hipStream_t st = (hipStream_t)stream;
   return out->dtype == HC_FLOAT64
-             ? frangi_eig_impl<double>(eig, ndim, out, alpha, beta, gamma, first_sigma, scratch, st)
-             : frangi_eig_impl<float>(eig, ndim, out, alpha, beta, gamma, first_sigma, scratch, st);
+             ? frangi_eig_impl<double>(eig, ndim, out, alpha, beta, gamma, has_gamma != 0,
+                                       first_sigma, scratch, st)
+             : frangi_eig_impl<float>(eig, ndim, out, alpha, beta, gamma, has_gamma != 0,
+                                      first_sigma, scratch, st);
 }
 
 int hc_frangi(const hc_array* in, const hc_array* out, const hc_taps* taps_s,
               const hc_taps* taps_d, int nsig, double alpha, double beta, double gamma,
-              int black_ridges, int mode, double cval, const hc_array* work, int nwork,
-              void* scratch, void* stream) {
+              int has_gamma, int black_ridges, int mode, double cval, const hc_array* work,
+              int nwork, void* scratch, void* stream) {
   int rc = check_float_image(in);
   if (rc != HC_OK) return rc;
   if (!out || !same_shape(in, out) || !is_c_contiguous(out))
@@ -1773,9 +1775,11 @@ int hc_frangi(const hc_array* in, const hc_array* out, const hc_taps* taps_s,
       return fail(HC_ERR_VALUE, "weights dtype must match the work dtype");
   hipStream_t st = (hipStream_t)stream;
   return f64 ? frangi_impl<double>(in, out, taps_s, taps_d, nsig, alpha, beta, gamma,
-                                   black_ridges, mode, cval, work, nwork, scratch, st)
+                                   has_gamma != 0, black_ridges, mode, cval, work, nwork,
+                                   scratch, st)
              : frangi_impl<float>(in, out, taps_s, taps_d, nsig, alpha, beta, gamma,
-                                  black_ridges, mode, cval, work, nwork, scratch, st);
+                                  has_gamma != 0, black_ridges, mode, cval, work, nwork,
+                                  scratch, st);
 }
 
 }  // extern "C"

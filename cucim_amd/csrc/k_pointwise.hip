@@ -528,7 +528,101 @@ __global__ __launch_bounds__(256) void k_binary(const F* a, const F* b, F* out, 
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * 256) {
     const F x = a[i], y = b[i];
-    out[i] = op == HC_BIN_MUL ? mul_rn(x, y) : add_rn(x, -y);
+    out[i] = op == HC_BIN_MUL ? mul_rn(x, y) : add_rn(x, op == HC_BIN_ADD ? y : -y);
+  }
+}
+
+template <typename F>
+__global__ __launch_bounds__(256) void k_div_scalar(F* __restrict__ a, int64_t n, F d) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256)
+    a[i] = a[i] / d;
+}
+
+// Sparse correlation (filters/_sparse.py:55-118): out[o] = sum over the
+// kernel's nonzero taps, in order, of w_k * P[o + idx_k], where P is the image
+// padded by pad_d on both sides of each axis (index remapping, constant mode
+// pads with 0).  Each product and each add is rounded once in F, as the
+// reference's `out += val * view` steps are; w = 1 and w = -1 take the view
+// (or its negation) without a multiply.  One launch handles a chunk of taps;
+// later chunks continue the sum from out.
+constexpr int kSparseTaps = 64;
+
+struct SparseArgs {
+  const void* in;
+  int in_dt;
+  void* out;
+  int ndim;
+  int64_t n;                       // output elements
+  int64_t oshape[HC_MAX_NDIM];     // output shape
+  int64_t ishape[HC_MAX_NDIM];     // input shape
+  int64_t istride[HC_MAX_NDIM];    // input element strides
+  int pad[HC_MAX_NDIM];            // padding before each axis
+  int mode;
+  int ntaps, first;                // first != 0: out starts at this chunk's tap 0
+  int16_t idx[kSparseTaps][HC_MAX_NDIM];
+  double w[kSparseTaps];
+};
+
+template <typename F>
+__device__ __forceinline__ F sparse_term(double w, F v) {
+  if (w == 1.0) return v;
+  if (w == -1.0) return -v;
+  return mul_rn((F)w, v);
+}
+
+template <typename F>
+__global__ __launch_bounds__(256) void k_correlate_sparse(const SparseArgs a) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < a.n;
+       i += (int64_t)gridDim.x * 256) {
+    int64_t o[HC_MAX_NDIM];
+    int64_t r = i;
+    for (int d = a.ndim - 1; d >= 0; --d) {
+      o[d] = r % a.oshape[d];
+      r /= a.oshape[d];
+    }
+    F s = a.first ? (F)0 : reinterpret_cast<F*>(a.out)[i];
+    for (int k = 0; k < a.ntaps; ++k) {
+      int64_t off = 0;
+      bool inside = true;
+      for (int d = 0; d < a.ndim; ++d) {
+        const int64_t q = remap(o[d] + a.idx[k][d] - a.pad[d], a.ishape[d], a.mode);
+        inside &= q >= 0;
+        off += (q >= 0 ? q : 0) * a.istride[d];
+      }
+      const F v = inside ? load_as<F>(a.in, off, a.in_dt) : (F)0;
+      const F t = sparse_term<F>(a.w[k], v);
+      s = (a.first && k == 0) ? t : add_rn(s, t);
+    }
+    reinterpret_cast<F*>(a.out)[i] = s;
+  }
+}
+
+// Integer element ops wrap modulo 2^bits like the in-place integer ufuncs:
+// U is the unsigned type of the element width, W an unsigned type at least
+// 32 bits wide (no promotion to signed int, so no overflow).
+template <typename U, typename W>
+__device__ __forceinline__ U wrap_op(U x, U y, int op) {
+  const W a = (W)x, b = (W)y;
+  return (U)(op == HC_BIN_MUL ? a * b : (op == HC_BIN_ADD ? a + b : a - b));
+}
+
+template <typename U, typename W>
+__global__ __launch_bounds__(256) void k_binary_int(const U* a, const U* b, U* out, int64_t n,
+                                                    int op) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256)
+    out[i] = wrap_op<U, W>(a[i], b[i], op);
+}
+
+template <typename U, typename W>
+__global__ __launch_bounds__(256) void k_square_acc_int(U* __restrict__ acc,
+                                                        const U* __restrict__ v, int64_t n,
+                                                        int first) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    const U s = wrap_op<U, W>(v[i], v[i], HC_BIN_MUL);
+    acc[i] = first ? s : wrap_op<U, W>(acc[i], s, HC_BIN_ADD);
   }
 }
 
@@ -537,6 +631,21 @@ __global__ __launch_bounds__(256) void k_sqrt(F* __restrict__ a, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * 256)
     a[i] = sqrt(a[i]);
+}
+
+// sqrt of an integer array in place with an unsafe cast back (sqrt(out, out,
+// casting='unsafe')): computed in the float type the ufunc picks for the
+// input (F: float for 8/16-bit, double for 32/64-bit; for 8-bit the ufunc's
+// float16 result truncates to the same integer as float's, since sqrt(k^2-1)
+// stays more than 1/32 below k), truncated toward zero; a negative input
+// gives NaN, stored as 0.
+template <typename T, typename F>
+__global__ __launch_bounds__(256) void k_sqrt_int(T* __restrict__ a, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    const F r = sqrt((F)a[i]);
+    a[i] = (r != r) ? (T)0 : (T)r;
+  }
 }
 
 struct CopyArgs {
@@ -865,15 +974,25 @@ int hc_square_acc(const hc_array* acc, const hc_array* v, int first, void* strea
     HC_LAUNCH("k_square_acc", st, k_square_acc<double>, dim3((unsigned)grid_for(n)), dim3(256), 0, (double*)acc->data, (const double*)v->data, n, first);
   else if (acc->dtype == HC_FLOAT32)
     HC_LAUNCH("k_square_acc", st, k_square_acc<float>, dim3((unsigned)grid_for(n)), dim3(256), 0, (float*)acc->data, (const float*)v->data, n, first);
-  else
-    return fail(HC_ERR_VALUE, "square_acc needs float32/float64");
+  else {
+    void* A = acc->data;
+    const void* V = v->data;
+    switch (dtype_size(acc->dtype) * (acc->dtype <= HC_UINT64 && acc->dtype != HC_BOOL)) {
+      case 1: HC_LAUNCH("k_square_acc_int", st, (k_square_acc_int<uint8_t, uint32_t>), dim3((unsigned)grid_for(n)), dim3(256), 0, (uint8_t*)A, (const uint8_t*)V, n, first); break;
+      case 2: HC_LAUNCH("k_square_acc_int", st, (k_square_acc_int<uint16_t, uint32_t>), dim3((unsigned)grid_for(n)), dim3(256), 0, (uint16_t*)A, (const uint16_t*)V, n, first); break;
+      case 4: HC_LAUNCH("k_square_acc_int", st, (k_square_acc_int<uint32_t, uint32_t>), dim3((unsigned)grid_for(n)), dim3(256), 0, (uint32_t*)A, (const uint32_t*)V, n, first); break;
+      case 8: HC_LAUNCH("k_square_acc_int", st, (k_square_acc_int<uint64_t, uint64_t>), dim3((unsigned)grid_for(n)), dim3(256), 0, (uint64_t*)A, (const uint64_t*)V, n, first); break;
+      default: return fail(HC_ERR_VALUE, "square_acc needs an integer or float32/float64 array");
+    }
+  }
   HC_HIP(hipGetLastError());
   return HC_OK;
 }
 
 int hc_binary(const hc_array* a, const hc_array* b, const hc_array* out, int op, void* stream) {
   if (!a || !b || !out) return fail(HC_ERR_VALUE, "null array");
-  if (op != HC_BIN_SUB && op != HC_BIN_MUL) return fail(HC_ERR_VALUE, "unknown binary op");
+  if (op != HC_BIN_SUB && op != HC_BIN_MUL && op != HC_BIN_ADD)
+    return fail(HC_ERR_VALUE, "unknown binary op");
   if (!same_shape(a, b) || !same_shape(a, out) || !is_c_contiguous(a) || !is_c_contiguous(b) ||
       !is_c_contiguous(out) || a->dtype != b->dtype || a->dtype != out->dtype)
     return fail(HC_ERR_VALUE, "arrays must be C-contiguous with equal shapes and dtypes");
@@ -886,9 +1005,94 @@ int hc_binary(const hc_array* a, const hc_array* b, const hc_array* out, int op,
   else if (a->dtype == HC_FLOAT32)
     HC_LAUNCH("k_binary", st, k_binary<float>, dim3((unsigned)grid_for(n)), dim3(256), 0,
               (const float*)a->data, (const float*)b->data, (float*)out->data, n, op);
-  else
-    return fail(HC_ERR_VALUE, "binary ops need float32/float64");
+  else {
+    const void* A = a->data;
+    const void* B = b->data;
+    void* O = out->data;
+    switch (dtype_size(a->dtype) * (a->dtype <= HC_UINT64 && a->dtype != HC_BOOL)) {
+      case 1: HC_LAUNCH("k_binary_int", st, (k_binary_int<uint8_t, uint32_t>), dim3((unsigned)grid_for(n)), dim3(256), 0, (const uint8_t*)A, (const uint8_t*)B, (uint8_t*)O, n, op); break;
+      case 2: HC_LAUNCH("k_binary_int", st, (k_binary_int<uint16_t, uint32_t>), dim3((unsigned)grid_for(n)), dim3(256), 0, (const uint16_t*)A, (const uint16_t*)B, (uint16_t*)O, n, op); break;
+      case 4: HC_LAUNCH("k_binary_int", st, (k_binary_int<uint32_t, uint32_t>), dim3((unsigned)grid_for(n)), dim3(256), 0, (const uint32_t*)A, (const uint32_t*)B, (uint32_t*)O, n, op); break;
+      case 8: HC_LAUNCH("k_binary_int", st, (k_binary_int<uint64_t, uint64_t>), dim3((unsigned)grid_for(n)), dim3(256), 0, (const uint64_t*)A, (const uint64_t*)B, (uint64_t*)O, n, op); break;
+      default: return fail(HC_ERR_VALUE, "binary ops need integer or float32/float64 arrays");
+    }
+  }
   HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+int hc_div_scalar(const hc_array* a, double d, void* stream) {
+  if (!a) return fail(HC_ERR_VALUE, "null array");
+  if (!is_c_contiguous(a)) return fail(HC_ERR_VALUE, "array must be C-contiguous");
+  const int64_t n = numel(a);
+  if (n == 0) return HC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (a->dtype == HC_FLOAT64)
+    HC_LAUNCH("k_div_scalar", st, k_div_scalar<double>, dim3((unsigned)grid_for(n)), dim3(256), 0, (double*)a->data, n, d);
+  else if (a->dtype == HC_FLOAT32)
+    HC_LAUNCH("k_div_scalar", st, k_div_scalar<float>, dim3((unsigned)grid_for(n)), dim3(256), 0, (float*)a->data, n, (float)d);
+  else
+    return fail(HC_ERR_VALUE, "division by a scalar needs float32/float64");
+  HC_HIP(hipGetLastError());
+  return HC_OK;
+}
+
+int hc_correlate_sparse(const hc_array* in, const hc_array* out, const int32_t* kshape,
+                        const int32_t* tap_idx, const double* tap_w, int ntaps, int valid,
+                        int mode, void* stream) {
+  if (!in || !out || !kshape || (ntaps > 0 && (!tap_idx || !tap_w)))
+    return fail(HC_ERR_VALUE, "null argument");
+  if (in->ndim != out->ndim || in->ndim < 1 || in->ndim > HC_MAX_NDIM)
+    return fail(HC_ERR_VALUE, "input and output ranks differ or are out of range");
+  if (out->dtype != HC_FLOAT32 && out->dtype != HC_FLOAT64)
+    return fail(HC_ERR_VALUE, "output must be float32 or float64");
+  if (!is_c_contiguous(out)) return fail(HC_ERR_VALUE, "output must be C-contiguous");
+  if (ntaps < 1) return fail(HC_ERR_VALUE, "at least one tap (the corner) is needed");
+  if (!valid && (mode < HC_MODE_REFLECT || mode > HC_MODE_CONSTANT))
+    return fail(HC_ERR_VALUE, "boundary mode not supported");
+  const int nd = in->ndim;
+  SparseArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = in->data;
+  a.in_dt = in->dtype;
+  a.out = out->data;
+  a.ndim = nd;
+  a.mode = valid ? HC_MODE_CONSTANT : mode;
+  for (int d = 0; d < nd; ++d)
+    if (kshape[d] < 1 || kshape[d] > 32767) return fail(HC_ERR_VALUE, "kernel shape out of range");
+  for (int k = 0; k < ntaps; ++k)
+    for (int d = 0; d < nd; ++d) {
+      const int32_t v = tap_idx[(int64_t)k * nd + d];
+      if (v < 0 || v >= kshape[d]) return fail(HC_ERR_VALUE, "tap index outside the kernel");
+    }
+  for (int d = 0; d < nd; ++d) {
+    a.ishape[d] = in->shape[d];
+    a.istride[d] = in->strides[d];
+    a.oshape[d] = out->shape[d];
+    // padded modes pad kshape // 2 on both sides (odd kernels keep the
+    // input's shape); 'valid' reads without padding and shrinks by kshape - 1
+    const int64_t want =
+        valid ? std::max<int64_t>(0, in->shape[d] - kshape[d] + 1) : in->shape[d];
+    if (!valid && kshape[d] % 2 == 0) return fail(HC_ERR_VALUE, "even kernel size");
+    if (out->shape[d] != want) return fail(HC_ERR_VALUE, "output shape does not match");
+    a.pad[d] = valid ? 0 : kshape[d] / 2;
+  }
+  a.n = numel(out);
+  if (a.n == 0) return HC_OK;
+  hipStream_t st = (hipStream_t)stream;
+  for (int k0 = 0; k0 < ntaps; k0 += kSparseTaps) {
+    a.ntaps = std::min(kSparseTaps, ntaps - k0);
+    a.first = k0 == 0;
+    for (int k = 0; k < a.ntaps; ++k) {
+      for (int d = 0; d < nd; ++d) a.idx[k][d] = (int16_t)tap_idx[(int64_t)(k0 + k) * nd + d];
+      a.w[k] = tap_w[k0 + k];
+    }
+    if (out->dtype == HC_FLOAT64)
+      HC_LAUNCH("k_correlate_sparse", st, k_correlate_sparse<double>, dim3((unsigned)grid_for(a.n)), dim3(256), 0, a);
+    else
+      HC_LAUNCH("k_correlate_sparse", st, k_correlate_sparse<float>, dim3((unsigned)grid_for(a.n)), dim3(256), 0, a);
+    HC_HIP(hipGetLastError());
+  }
   return HC_OK;
 }
 
@@ -902,8 +1106,21 @@ int hc_sqrt_inplace(const hc_array* a, void* stream) {
     HC_LAUNCH("k_sqrt", st, k_sqrt<double>, dim3((unsigned)grid_for(n)), dim3(256), 0, (double*)a->data, n);
   else if (a->dtype == HC_FLOAT32)
     HC_LAUNCH("k_sqrt", st, k_sqrt<float>, dim3((unsigned)grid_for(n)), dim3(256), 0, (float*)a->data, n);
-  else
-    return fail(HC_ERR_VALUE, "sqrt needs float32/float64");
+  else {
+    void* A = a->data;
+    const dim3 g((unsigned)grid_for(n));
+    switch (a->dtype) {
+      case HC_INT8: HC_LAUNCH("k_sqrt_int", st, (k_sqrt_int<int8_t, float>), g, dim3(256), 0, (int8_t*)A, n); break;
+      case HC_UINT8: HC_LAUNCH("k_sqrt_int", st, (k_sqrt_int<uint8_t, float>), g, dim3(256), 0, (uint8_t*)A, n); break;
+      case HC_INT16: HC_LAUNCH("k_sqrt_int", st, (k_sqrt_int<int16_t, float>), g, dim3(256), 0, (int16_t*)A, n); break;
+      case HC_UINT16: HC_LAUNCH("k_sqrt_int", st, (k_sqrt_int<uint16_t, float>), g, dim3(256), 0, (uint16_t*)A, n); break;
+      case HC_INT32: HC_LAUNCH("k_sqrt_int", st, (k_sqrt_int<int32_t, double>), g, dim3(256), 0, (int32_t*)A, n); break;
+      case HC_UINT32: HC_LAUNCH("k_sqrt_int", st, (k_sqrt_int<uint32_t, double>), g, dim3(256), 0, (uint32_t*)A, n); break;
+      case HC_INT64: HC_LAUNCH("k_sqrt_int", st, (k_sqrt_int<int64_t, double>), g, dim3(256), 0, (int64_t*)A, n); break;
+      case HC_UINT64: HC_LAUNCH("k_sqrt_int", st, (k_sqrt_int<uint64_t, double>), g, dim3(256), 0, (uint64_t*)A, n); break;
+      default: return fail(HC_ERR_VALUE, "sqrt needs an integer or float32/float64 array");
+    }
+  }
   HC_HIP(hipGetLastError());
   return HC_OK;
 }

@@ -146,18 +146,25 @@ def hessian_matrix(image, sigma=1, mode="constant", cval=0, order="rc",
 
 
 def _symmetric_compute_eigenvalues(S_elems, sort="descending", abs_sort=False):
-    """``corner.py:699-755``: closed form for 2-D / 3-D, returned as one
-    (ndim,) + shape array."""
+    """``corner.py:699-755``: closed form for 2-D / 3-D, the eigvalsh branch
+    (Jacobi rotations on the device) for other orders up to 8; returned as
+    one (ndim,) + shape array."""
     S = [_array.as_tensor(s) for s in S_elems]
     if len(S) not in (3, 6):
-        raise NotImplementedError("eigenvalues of more than 3x3 matrices are not supported")
+        # the eigvalsh branch builds matrices of the image's rank from the
+        # first n(n+1)/2 elements (_symmetric_image, corner.py:758-783)
+        ne = S[0].dim()
+        if not 1 <= ne <= 8:
+            raise ValueError("eigenvalues of matrices of order 1..8 only")
+        if len(S) < ne * (ne + 1) // 2:
+            raise ValueError(f"{ne}-D images need {ne * (ne + 1) // 2} matrix elements")
+        S = S[:ne * (ne + 1) // 2]
     if _array.np_dtype(S[0].dtype).kind != "f":
         raise ValueError("expected real-valued floating point matrices")
     if sort not in ("ascending", "descending"):
         raise ValueError(f"unknown sort type: {sort}")
     dt = S[0].dtype
     S = [s.to(dt).contiguous() for s in S]
-    ne = 2 if len(S) == 3 else 3
     if dt not in (_array.torch_dtype(np.float32), _array.torch_dtype(np.float64)):
         S = [astype(s, np.float32) for s in S]
         dt = S[0].dtype

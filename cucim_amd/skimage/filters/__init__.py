@@ -2,5 +2,23 @@
 
 from .._shared.filters import gaussian  # noqa: F401
 from ._gaussian import difference_of_gaussians  # noqa: F401
-from .edges import farid, prewitt, scharr, sobel  # noqa: F401
+from ._sparse import correlate_sparse  # noqa: F401
+from .edges import (  # noqa: F401
+    farid,
+    farid_h,
+    farid_v,
+    laplace,
+    prewitt,
+    prewitt_h,
+    prewitt_v,
+    roberts,
+    roberts_neg_diag,
+    roberts_pos_diag,
+    scharr,
+    scharr_h,
+    scharr_v,
+    sobel,
+    sobel_h,
+    sobel_v,
+)
 from .ridges import frangi, hessian, meijering, sato  # noqa: F401

@@ -122,7 +122,140 @@ def prewitt(image, mask=None, *, axis=None, mode="reflect", cval=0.0):
 
 
 def farid(image, mask=None, *, axis=None, mode="reflect", cval=0.0):
-    """Farid edge map (5-tap derivative / smoothing pair)."""
+    """Farid edge map, 5-tap derivative / smoothing pair (``edges.py:783-850``)."""
     out = _generic_edge_filter(image, smooth_weights=farid_smooth, edge_weights=farid_edge,
                                axis=axis, mode=mode, cval=cval)
     return _mask_filter_result(out, mask)
+
+
+def _edge_input(image):
+    """Float conversion of the 2-D-only edge filters (``edges.py:680-684``):
+    float images keep their supported float type, others go through
+    img_as_float."""
+    image = _array.as_tensor(image)
+    if _array.np_dtype(image.dtype).kind == "f":
+        return astype(image, _supported_float_type(image.dtype))
+    return img_as_float(image)
+
+
+def _check_2d(image):
+    from .._shared.utils import check_nD
+
+    check_nD(_array.as_tensor(image), 2)
+
+
+def sobel_h(image, mask=None):
+    """Horizontal edges: sobel along axis 0 (``edges.py:302-329``)."""
+    _check_2d(image)
+    return sobel(image, mask=mask, axis=0)
+
+
+def sobel_v(image, mask=None):
+    """Vertical edges: sobel along axis 1 (``edges.py:332-359``)."""
+    _check_2d(image)
+    return sobel(image, mask=mask, axis=1)
+
+
+def scharr_h(image, mask=None):
+    """Horizontal edges: scharr along axis 0 (``edges.py:425-457``)."""
+    _check_2d(image)
+    return scharr(image, mask=mask, axis=0)
+
+
+def scharr_v(image, mask=None):
+    """Vertical edges: scharr along axis 1 (``edges.py:460-491``)."""
+    _check_2d(image)
+    return scharr(image, mask=mask, axis=1)
+
+
+def prewitt_h(image, mask=None):
+    """Horizontal edges: prewitt along axis 0 (``edges.py:554-581``)."""
+    _check_2d(image)
+    return prewitt(image, mask=mask, axis=0)
+
+
+def prewitt_v(image, mask=None):
+    """Vertical edges: prewitt along axis 1 (``edges.py:584-611``)."""
+    _check_2d(image)
+    return prewitt(image, mask=mask, axis=1)
+
+
+# ``edges.py:50-53``: 2-D Roberts cross kernels (convolution weights)
+ROBERTS_PD_WEIGHTS = np.array([[1, 0], [0, -1]], dtype=np.float64)
+ROBERTS_ND_WEIGHTS = np.array([[0, 1], [-1, 0]], dtype=np.float64)
+# ``edges.py:71-72``: full 5x5 Farid kernels (edge along axis 0 for _h)
+HFARID_WEIGHTS = farid_edge[:, None] * farid_smooth[None, :]
+VFARID_WEIGHTS = np.copy(HFARID_WEIGHTS.T)
+
+
+def _convolve_masked(image, weights, mask):
+    """n-D convolution with the weights cast to the image's float dtype, then
+    the mask (``edges.py:659-738, 853-927``)."""
+    image = _edge_input(image)
+    w = np.asarray(weights, dtype=_array.np_dtype(image.dtype))
+    result = ndi.convolve(image, w)
+    return _mask_filter_result(result, mask)
+
+
+def roberts_pos_diag(image, mask=None):
+    """Roberts cross along the positive diagonal (``edges.py:659-696``)."""
+    _check_2d(image)
+    return _convolve_masked(image, ROBERTS_PD_WEIGHTS, mask)
+
+
+def roberts_neg_diag(image, mask=None):
+    """Roberts cross along the negative diagonal (``edges.py:699-736``)."""
+    _check_2d(image)
+    return _convolve_masked(image, ROBERTS_ND_WEIGHTS, mask)
+
+
+def roberts(image, mask=None):
+    """Roberts edge magnitude (``edges.py:614-660``): each step rounded in
+    the float dtype as the reference's in-place ufuncs do (``edges.py:614-656``):
+    pos *= pos; out = neg * neg; out += pos; sqrt; out /= sqrt(2)."""
+    _check_2d(image)
+    pos = roberts_pos_diag(image, mask)
+    out = roberts_neg_diag(image, mask)
+    lib = _lib.lib()
+    st = _array.stream()
+    _lib.check(lib.hc_square_acc(_array.hc_array(out), _array.hc_array(out), 1, st), "roberts")
+    _lib.check(lib.hc_square_acc(_array.hc_array(out), _array.hc_array(pos), 0, st), "roberts")
+    _lib.check(lib.hc_sqrt_inplace(_array.hc_array(out), st), "roberts")
+    _lib.check(lib.hc_div_scalar(_array.hc_array(out), math.sqrt(2), st), "roberts")
+    return out
+
+
+def _laplacian_kernel(ndim, dtype):
+    """The spatial kernel of ``restoration/uft.py:429-478`` (``laplacian``
+    with shape None): 3^ndim zeros, -1 at the two axial neighbours of the
+    centre along every axis, 2*ndim at the centre."""
+    k = np.zeros([3] * ndim)
+    for d in range(ndim):
+        idx = [1] * ndim
+        for v in (0, 2):
+            idx[d] = v
+            k[tuple(idx)] = -1.0
+    k[(1,) * ndim] = 2.0 * ndim
+    return k.astype(dtype)
+
+
+def laplace(image, ksize=3, mask=None):
+    """Discrete Laplace operator (``edges.py:739-780``)."""
+    image = _edge_input(image)
+    if ksize != 3:
+        raise NotImplementedError("only ksize=3 is supported")
+    k = _laplacian_kernel(image.dim(), _array.np_dtype(image.dtype))
+    result = ndi.convolve(image, k)
+    return _mask_filter_result(result, mask)
+
+
+def farid_h(image, *, mask=None):
+    """Horizontal Farid edges: the full 5x5 kernel (``edges.py:853-890``)."""
+    _check_2d(image)
+    return _convolve_masked(image, HFARID_WEIGHTS, mask)
+
+
+def farid_v(image, *, mask=None):
+    """Vertical Farid edges (``edges.py:893-927``)."""
+    _check_2d(image)
+    return _convolve_masked(image, VFARID_WEIGHTS, mask)

@@ -49,8 +49,8 @@ class FrangiPlan:
         x = image.contiguous()
         return _lib.lib().hc_frangi(
             _array.hc_array(x), _array.hc_array(out), self.ts, self.td, len(self.sigmas),
-            float(alpha), float(beta), -1.0 if gamma is None else float(gamma),
-            1 if black_ridges else 0, _mode_code(self.mode), float(cval), self.work_hc,
+            float(alpha), float(beta), 0.0 if gamma is None else float(gamma),
+            0 if gamma is None else 1, 1 if black_ridges else 0, _mode_code(self.mode), float(cval), self.work_hc,
             len(self.work), self.scratch.data_ptr(), _array.stream())
 
 
@@ -69,7 +69,8 @@ def _frangi_sequential(image, sigmas, alpha, beta, gamma, black_ridges, mode, cv
         E = [eig[k] for k in range(img.dim())]
         rc = lib.hc_frangi_from_eigvals(
             _array.hc_arrays(E), img.dim(), _array.hc_array(out), float(alpha), float(beta),
-            -1.0 if gamma is None else float(gamma), 1 if i == 0 else 0,
+            0.0 if gamma is None else float(gamma), 0 if gamma is None else 1,
+            1 if i == 0 else 0,
             scratch.data_ptr(), _array.stream())
         _lib.check(rc, "frangi")
     return out
@@ -121,6 +122,7 @@ def sato(image, sigmas=range(1, 10, 2), black_ridges=True, mode="reflect", cval=
     mean of the positive Hessian eigenvalues other than the smallest, times
     sigma^2; maximum over sigmas (from 0)."""
     check_nD(_array.as_tensor(image), [2, 3])
+    sigmas = [float(s) for s in sigmas]  # one pass over a one-shot iterable
     image, float_dtype = _ridge_input(image, black_ridges)
     out = _array.empty(tuple(image.shape), float_dtype)
     lib = _lib.lib()
@@ -128,7 +130,7 @@ def sato(image, sigmas=range(1, 10, 2), black_ridges=True, mode="reflect", cval=
         H = hessian_matrix(image, sigma, mode=mode, cval=cval, use_gaussian_derivatives=True)
         _lib.check(lib.hc_sato_fold(_array.hc_arrays(H), len(H), _array.hc_array(out),
                                     float(sigma), 1 if i == 0 else 0, _array.stream()), "sato")
-    if len(list(sigmas)) == 0:
+    if not sigmas:
         out.zero_()
     return out
 

@@ -191,32 +191,36 @@ int hc_hessian_gaussian(const hc_array* in, const hc_array* H, int nH,
                         const hc_taps* taps_s, const hc_taps* taps_d, int mode,
                         double cval, int order_xy, void* stream);
 
-/* Closed-form symmetric 2x2 / 3x3 eigenvalues: feature/corner.py:473-696
- * (_symmetric_compute_eigenvalues).  S: 3 or 6 element arrays, eig: ndim
- * outputs.  sort: 0 ascending, 1 descending. */
+/* Symmetric eigenvalues per voxel: feature/corner.py:699-755
+ * (_symmetric_compute_eigenvalues).  S: nS = n(n+1)/2 element arrays in
+ * combinations_with_replacement order (n = 1..8), eig: n outputs.  n = 2, 3:
+ * the closed forms of :473-696; other n: the eigvalsh branch (:744-755,
+ * _symmetric_image :758-783), by cyclic Jacobi rotations in float64.
+ * sort: 0 ascending, 1 descending; abs_sort orders by |value| first. */
 int hc_sym_eigvals(const hc_array* S, int nS, const hc_array* eig, int sort,
                    int abs_sort, void* stream);
 
 /* Frangi vesselness: filters/ridges.py:322-476.  taps_s/taps_d hold
- * nsig*ndim kernels (sigma-major).  gamma < 0 means "None" (gamma from the
- * first sigma, computed on the device; no host synchronisation).  work: 6
+ * nsig*ndim kernels (sigma-major).  has_gamma == 0 means gamma=None (gamma
+ * from the first sigma, computed on the device; no host synchronisation);
+ * otherwise gamma is used as given (any value, 2*gamma^2 as the reference).  work: 6
  * (3-D) or 3 (2-D) arrays of the image's shape in the work dtype;
  * scratch: >= 64 bytes of device memory.  out: image shape, work dtype.
  * mode='constant' with cval != 0 returns HC_ERR_UNSUPPORTED (see
  * hc_hessian_gaussian). */
 int hc_frangi(const hc_array* in, const hc_array* out, const hc_taps* taps_s,
               const hc_taps* taps_d, int nsig, double alpha, double beta,
-              double gamma, int black_ridges, int mode, double cval,
+              double gamma, int has_gamma, int black_ridges, int mode, double cval,
               const hc_array* work, int nwork, void* scratch, void* stream);
 
 /* Frangi vesselness of one sigma from eigenvalues already sorted ascending
  * by |value| (filters/ridges.py:428-475 after the eigenvalue step): eig holds
  * ndim arrays; first_sigma != 0 writes out, else out = max(out, v).  With
- * gamma < 0, the first sigma reduces max(sum lambda^2) into scratch
+ * has_gamma == 0, the first sigma reduces max(sum lambda^2) into scratch
  * (>= 64 bytes) and later sigmas read gamma from there.  Serves images whose
  * lines do not fit the fused hc_frangi schedule. */
 int hc_frangi_from_eigvals(const hc_array* eig, int ndim, const hc_array* out,
-                           double alpha, double beta, double gamma,
+                           double alpha, double beta, double gamma, int has_gamma,
                            int first_sigma, void* scratch, void* stream);
 
 /* Sato tubeness of one sigma folded into the running maximum over sigmas:
@@ -277,7 +281,9 @@ int hc_cnn3d_adam(float* p, const float* g, float* m, float* v, int64_t n, doubl
  * border; per sigma kc holds the composite kernels SS, SD, DD (full
  * convolutions of the pass kernels, L taps each, anchor A = 2 * anchor) and
  * kinfo (offset, L, A).  plain_out: nb x 64 values; err_out: per line
- * sum (sur - plain)^2, sum plain^2 and the number of samples. */
+ * sum (sur - plain)^2, sum plain^2 and the number of samples.  A line with
+ * zb - za outside [0, 64], or whose windows would reach outside the volume,
+ * is not evaluated (0 samples, NaN plain values). */
 int hc_frangi_check(const hc_array* vol, const hc_array* sur, const int32_t* lines_dev, int nb,
                     const float* kc_dev, const int32_t* kinfo_dev, const int32_t* kinfo_host,
                     int nsig, int black_ridges, double alpha, double beta, double gamma,
@@ -313,18 +319,43 @@ int hc_mask_eroded(const hc_array* out, const hc_array* mask, void* stream);
 /* Elementwise NaN-propagating maximum a = max(a, b) (cp.maximum). */
 int hc_maximum_inplace(const hc_array* a, const hc_array* b, void* stream);
 
-/* acc = v*v (first != 0) or acc += v*v, float32/float64, C-contiguous. */
+/* acc = v*v (first != 0) or acc += v*v, each op rounded once (float32 /
+ * float64) or wrapped (integer dtypes), C-contiguous: the `output *= output;
+ * tmp *= tmp; output += tmp` of _vendored/_ndimage_filters.py:1012-1097. */
 int hc_square_acc(const hc_array* acc, const hc_array* v, int first, void* stream);
 
-/* a = sqrt(a), float32/float64, C-contiguous. */
+/* a = sqrt(a), C-contiguous; integer arrays as sqrt(a, a, casting='unsafe')
+ * (float for 8/16-bit, double for wider, truncated; NaN -> 0). */
 int hc_sqrt_inplace(const hc_array* a, void* stream);
 
-/* Elementwise out = a - b (HC_BIN_SUB) or a * b (HC_BIN_MUL), each result
- * rounded once in the arrays' dtype (float32/float64; equal shapes, dtypes,
- * C-contiguous; out may alias a or b).  The `out -= gaussian(...)` of
- * filters/_gaussian.py:154-172 and the `der0 * der1` products of
- * feature/corner.py:137-148. */
-enum { HC_BIN_SUB = 0, HC_BIN_MUL = 1 };
+/* Sparse n-D correlation: filters/_sparse.py:94-118 (correlate_sparse) with
+ * _correlate_sparse :55-91.  kshape: the kernel's shape (ndim values);
+ * tap_idx (ntaps x ndim, host) / tap_w (ntaps, host, already rounded to the
+ * output dtype): the nonzero taps in C order, led by the corner (0, ..., 0)
+ * (value 0 when the corner tap is zero, as the reference prepends it).
+ * valid != 0: no padding, out shape = in shape - kshape + 1; otherwise the
+ * image is padded by kshape // 2 per side in `mode` (HC_MODE_*, constant
+ * pads with 0; even kernel sizes are rejected) and out has in's shape.
+ * out (float32/float64, C-contiguous) = sum_k w_k * P[o + idx_k], each
+ * product and add rounded once in out's dtype (w = +-1: no multiply).  in:
+ * any real dtype and strides, converted to out's dtype. */
+int hc_correlate_sparse(const hc_array* in, const hc_array* out, const int32_t* kshape,
+                        const int32_t* tap_idx, const double* tap_w, int ntaps, int valid,
+                        int mode, void* stream);
+
+/* a /= d in place, d rounded to a's dtype first and each quotient rounded
+ * once (float32/float64, C-contiguous): the `out /= math.sqrt(2)` of
+ * filters/edges.py:614-660 (roberts). */
+int hc_div_scalar(const hc_array* a, double d, void* stream);
+
+/* Elementwise out = a - b (HC_BIN_SUB), a * b (HC_BIN_MUL) or a + b
+ * (HC_BIN_ADD), each result rounded once in the arrays' dtype (float32 /
+ * float64), or wrapped modulo 2^bits for integer dtypes (equal shapes,
+ * dtypes, C-contiguous; out may alias a or b).  The `out -= gaussian(...)` of
+ * filters/_gaussian.py:154-172, the `der0 * der1` products of
+ * feature/corner.py:137-148 and the `output += tmp` of
+ * _vendored/_ndimage_filters.py:806-890 (generic_laplace). */
+enum { HC_BIN_SUB = 0, HC_BIN_MUL = 1, HC_BIN_ADD = 2 };
 int hc_binary(const hc_array* a, const hc_array* b, const hc_array* out, int op, void* stream);
 
 /* Strided copy with dtype cast (cast<> semantics of

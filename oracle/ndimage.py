@@ -517,3 +517,128 @@ def correlate(input, weights, output=None, mode="reflect", cval=0.0, origin=0,
         else:
             s = s + vals * wk
     return cast_to(s, out_dtype)
+
+
+def _wrap_or_round(a, b, op):
+    """``output += tmp`` / ``output *= output`` of CuPy's in-place ufuncs on
+    arrays of one dtype: one rounding per element for floats, wrap-around
+    modulo 2**bits for integers (numpy's integer array ops wrap too)."""
+    with np.errstate(over="ignore", invalid="ignore"):
+        r = a + b if op == "add" else a * b
+    return r.astype(a.dtype)
+
+
+def _sqrt_unsafe(a):
+    """``cupy.sqrt(output, output, casting='unsafe')``
+    (``_ndimage_filters.py:1097``): floats in place; integers through the
+    float type the ufunc picks (float16 for 8-bit, float32 for 16-bit,
+    float64 wider), truncated back, with the NaN of a negative value stored
+    as 0 (the device conversion)."""
+    a = np.asarray(a)
+    if a.dtype.kind == "f":
+        with np.errstate(invalid="ignore"):
+            return np.sqrt(a)
+    F = {1: np.float16, 2: np.float32}.get(a.dtype.itemsize, np.float64)
+    with np.errstate(invalid="ignore"):
+        r = np.sqrt(a.astype(F))
+    r = np.where(np.isnan(r), 0, r)
+    return np.trunc(r).astype(a.dtype)
+
+
+def _axes(axes, ndim):
+    if axes is None:
+        return tuple(range(ndim))
+    if np.isscalar(axes):
+        axes = (axes,)
+    return tuple(int(a) % ndim for a in axes)
+
+
+def generic_laplace_arrays(derivs):
+    """``_ndimage_filters.py:806-890``: the first axis's derivative is the
+    output, the others are added in, in axis order."""
+    out = derivs[0]
+    for t in derivs[1:]:
+        out = _wrap_or_round(out, t, "add")
+    return out
+
+
+def generic_gradient_magnitude_arrays(derivs):
+    """``_ndimage_filters.py:1012-1097``: output *= output, then per further
+    axis tmp *= tmp; output += tmp; finally the unsafe in-place sqrt."""
+    out = _wrap_or_round(derivs[0], derivs[0], "mul")
+    for t in derivs[1:]:
+        out = _wrap_or_round(out, _wrap_or_round(t, t, "mul"), "add")
+    return _sqrt_unsafe(out)
+
+
+def laplace(input, output=None, mode="reflect", cval=0.0, *, axes=None, acc="f64"):
+    """``_ndimage_filters.py:893-930``: correlate1d with [1, -2, 1] (weights of
+    dtype promote(input, float32)) along each axis, summed in the output
+    dtype."""
+    x = np.asarray(input)
+    out_dtype = x.dtype if output is None else np.dtype(output)
+    ax = _axes(axes, x.ndim)
+    if len(ax) == 0:
+        return cast_to(x, out_dtype) if out_dtype.kind != "f" else x.astype(out_dtype)
+    modes = _fix_sequence_arg(mode, len(ax), "mode", str)
+    wdt = np.float64 if acc == "f64" else np.promote_types(x.dtype, np.float32)
+    w = np.array([1, -2, 1], dtype=wdt)
+    derivs = [correlate1d(x, w, a, out_dtype, m, cval, acc=acc) for a, m in zip(ax, modes)]
+    return generic_laplace_arrays(derivs)
+
+
+def gaussian_laplace(input, sigma, output=None, mode="reflect", cval=0.0, *, axes=None,
+                     truncate=4.0, acc="f64"):
+    """``_ndimage_filters.py:933-1009``: per axis a, gaussian_filter with
+    order 2 along a and 0 elsewhere (sigma on the listed axes, 0 on the
+    others when ``axes`` is a subset), summed in the output dtype."""
+    x = np.asarray(input)
+    out_dtype = x.dtype if output is None else np.dtype(output)
+    ax = _axes(axes, x.ndim)
+    if len(ax) == 0:
+        return cast_to(x, out_dtype) if out_dtype.kind != "f" else x.astype(out_dtype)
+    modes = _fix_sequence_arg(mode, len(ax), "mode", str)
+    sig = _fix_sequence_arg(sigma, len(ax), "sigma", float)
+    if len(ax) < x.ndim:
+        full = [0.0] * x.ndim
+        for s, a in zip(sig, ax):
+            full[a] = s
+        sig = full
+    derivs = []
+    for a, m in zip(ax, modes):
+        order = [0] * x.ndim
+        order[a] = 2
+        derivs.append(gaussian_filter(x, sig, order, out_dtype, m, cval, truncate, acc=acc))
+    return generic_laplace_arrays(derivs)
+
+
+def gaussian_gradient_magnitude(input, sigma, output=None, mode="reflect", cval=0.0, *,
+                                axes=None, truncate=4.0, acc="f64"):
+    """``_ndimage_filters.py:1100-1157``: per axis a, gaussian_filter with
+    order 1 along a (sigma as given, over every axis), combined by
+    generic_gradient_magnitude."""
+    x = np.asarray(input)
+    out_dtype = x.dtype if output is None else np.dtype(output)
+    ax = _axes(axes, x.ndim)
+    if x.ndim == 0:
+        return x.astype(out_dtype)
+    modes = _fix_sequence_arg(mode, len(ax), "mode", str)
+    derivs = []
+    for a, m in zip(ax, modes):
+        order = [0] * x.ndim
+        order[a] = 1
+        derivs.append(gaussian_filter(x, sigma, order, out_dtype, m, cval, truncate, acc=acc))
+    return generic_gradient_magnitude_arrays(derivs)
+
+
+def convolve(input, weights, output=None, mode="reflect", cval=0.0, origin=0, *,
+             acc="f64"):
+    """n-D convolution (``_ndimage_filters.py:85-132, 236-243``): the weights
+    reversed along every axis and, per axis, origin -> -origin, minus 1 more
+    for an even-sized axis; then ``correlate``."""
+    x = np.asarray(input)
+    w = np.asarray(weights)
+    origins = _fix_sequence_arg(origin, x.ndim, "origin", int)
+    wr = w[(slice(None, None, -1),) * w.ndim]
+    o2 = [-o - (1 if s % 2 == 0 else 0) for o, s in zip(origins, w.shape)]
+    return correlate(x, wr, output, mode, cval, o2, acc=acc)

@@ -13,6 +13,9 @@ Reference files (relative to ``python/cucim/src/cucim/skimage/``):
   _shared/utils.py:1019-1179       convert_to_float, _supported_float_type
   util/dtype.py:219-517            img_as_float32/64 (_convert)
   filters/edges.py:33-299          sobel / prewitt / _generic_edge_filter
+  filters/edges.py:362-927         scharr, farid, roberts*, laplace, farid_h/v
+  restoration/uft.py:429-478       laplacian (spatial kernel)
+  filters/_sparse.py:34-118        correlate_sparse
   color/colorconv.py:160-180,1037-1095  rgb2gray
   feature/corner.py:161-856        hessian_matrix, eigenvalue kernels
   filters/ridges.py:228-476        frangi
@@ -41,6 +44,15 @@ __all__ = [
     "gaussian",
     "sobel",
     "prewitt",
+    "scharr",
+    "farid",
+    "roberts",
+    "roberts_pos_diag",
+    "roberts_neg_diag",
+    "laplace",
+    "farid_h",
+    "farid_v",
+    "correlate_sparse",
     "rgb2gray",
     "gradient",
     "hessian_matrix",
@@ -200,6 +212,152 @@ def prewitt(image, mask=None, *, axis=None, mode="reflect", cval=0.0, ndi=None,
         fp = sndi.generate_binary_structure(mask.ndim, mask.ndim)
         m = sndi.binary_erosion(np.asarray(mask), fp, border_value=0)
         out = out * m
+    return out
+
+
+def _mask_result(out, mask):
+    """``filters/edges.py:75-85``: multiply by the mask eroded with a full
+    structure (border value 0)."""
+    if mask is None:
+        return out
+    import scipy.ndimage as sndi
+
+    m = np.asarray(mask)
+    fp = sndi.generate_binary_structure(m.ndim, m.ndim)
+    m = sndi.binary_erosion(m, fp, border_value=0)
+    return (out * m).astype(out.dtype)
+
+
+def scharr(image, mask=None, *, axis=None, mode="reflect", cval=0.0, ndi=None,
+           dtype=None):
+    """``filters/edges.py:362-420`` (weights ``:37-40``)."""
+    out = _generic_edge_filter(image, smooth_weights=np.array([3, 10, 3]) / 16,
+                               axis=axis, mode=mode, cval=cval, ndi=ndi, dtype=dtype)
+    return _mask_result(out, mask)
+
+
+# ``filters/edges.py:60-69``: Farid & Simoncelli (2004) 5-tap pair
+FARID_SMOOTH = np.array([0.0376593171958126, 0.249153396177344, 0.426374573253687,
+                         0.249153396177344, 0.0376593171958126])
+FARID_EDGE = np.array([0.109603762960254, 0.276690988455557, 0, -0.276690988455557,
+                       -0.109603762960254])
+
+
+def farid(image, mask=None, *, axis=None, mode="reflect", cval=0.0, ndi=None, dtype=None):
+    """``filters/edges.py:783-850``."""
+    out = _generic_edge_filter(image, smooth_weights=FARID_SMOOTH, edge_weights=FARID_EDGE,
+                               axis=axis, mode=mode, cval=cval, ndi=ndi, dtype=dtype)
+    return _mask_result(out, mask)
+
+
+def _edge_float(image):
+    """``filters/edges.py:680-684``: float images keep their supported float
+    type, others go through img_as_float."""
+    image = np.asarray(image)
+    if image.dtype.kind == "f":
+        return image.astype(supported_float_type(image.dtype), copy=False)
+    return img_as_float(image)
+
+
+def _conv_masked(image, weights, mask, acc):
+    x = _edge_float(image)
+    w = np.asarray(weights, dtype=x.dtype)
+    return _mask_result(ondi.convolve(x, w, acc=acc), mask)
+
+
+def roberts_pos_diag(image, mask=None, *, acc="f64"):
+    """``filters/edges.py:659-696``: convolve with [[1, 0], [0, -1]]."""
+    return _conv_masked(image, [[1, 0], [0, -1]], mask, acc)
+
+
+def roberts_neg_diag(image, mask=None, *, acc="f64"):
+    """``filters/edges.py:699-736``: convolve with [[0, 1], [-1, 0]]."""
+    return _conv_masked(image, [[0, 1], [-1, 0]], mask, acc)
+
+
+def roberts(image, mask=None, *, acc="f64"):
+    """``filters/edges.py:614-656``: pos *= pos; out = neg * neg; out += pos;
+    sqrt(out); out /= sqrt(2), each step in the float dtype."""
+    pos = roberts_pos_diag(image, mask, acc=acc)
+    neg = roberts_neg_diag(image, mask, acc=acc)
+    F = pos.dtype.type
+    out = (neg * neg).astype(F) + (pos * pos).astype(F)
+    out = np.sqrt(out.astype(F))
+    return (out / F(math.sqrt(2))).astype(F)
+
+
+def laplacian_kernel(ndim, dtype=np.float64):
+    """``restoration/uft.py:429-478`` with shape None: -1 at the axial
+    neighbours of the centre of a 3^ndim block, 2*ndim at the centre."""
+    k = np.zeros([3] * ndim)
+    c = (1,) * ndim
+    for d in range(ndim):
+        for v in (0, 2):
+            idx = list(c)
+            idx[d] = v
+            k[tuple(idx)] = -1.0
+    k[c] = 2.0 * ndim
+    return k.astype(dtype)
+
+
+def laplace(image, ksize=3, mask=None, *, acc="f64"):
+    """``filters/edges.py:739-780``."""
+    x = _edge_float(image)
+    if ksize != 3:
+        raise NotImplementedError("only ksize=3 is supported")
+    return _mask_result(ondi.convolve(x, laplacian_kernel(x.ndim, x.dtype), acc=acc), mask)
+
+
+def farid_h(image, *, mask=None, acc="f64"):
+    """``filters/edges.py:853-890``: the 5x5 kernel edge (axis 0) x smooth."""
+    return _conv_masked(image, FARID_EDGE[:, None] * FARID_SMOOTH[None, :], mask, acc)
+
+
+def farid_v(image, *, mask=None, acc="f64"):
+    """``filters/edges.py:893-927``: the transposed kernel."""
+    return _conv_masked(image, (FARID_EDGE[:, None] * FARID_SMOOTH[None, :]).T, mask, acc)
+
+
+def correlate_sparse(image, kernel, mode="reflect"):
+    """``filters/_sparse.py:94-118`` with ``_correlate_sparse`` :55-91 and
+    ``_get_view`` :34-52: numpy.pad by kernel.shape // 2 (modes translated
+    by ``_shared/utils.py:1095-1102``; 'valid' pads nothing), then the views
+    at the nonzero taps (C order, led by the corner tap, value 0 if it is
+    zero) scaled and summed with one rounding per step in the supported
+    float type."""
+    image = np.asarray(image)
+    F = supported_float_type(image.dtype).type
+    image = image.astype(F, copy=False)
+    k = np.asarray(kernel)
+    if mode == "valid":
+        padded = image
+    else:
+        if any(w % 2 == 0 for w in k.shape):
+            raise ValueError("Window size must not be even on any dimension")
+        np_mode = {"nearest": "edge", "reflect": "symmetric", "mirror": "reflect"}.get(mode, mode)
+        padded = np.pad(image, [(w // 2, w // 2) for w in k.shape], mode=np_mode)
+    idx = np.nonzero(k)
+    values = list(k[idx].astype(F))
+    taps = list(zip(*idx))
+    corner = (0,) * k.ndim
+    if corner not in taps:
+        taps = [corner] + taps
+        values = [F(0.0)] + values
+
+    def view(t, v):
+        sl = tuple(slice(c, s - (w - 1 - c)) for c, w, s in zip(t, k.shape, padded.shape))
+        x = padded[sl]
+        if v == 1:
+            return x.copy()
+        if v == -1:
+            return -x
+        with np.errstate(invalid="ignore", over="ignore"):
+            return (F(v) * x).astype(F)
+
+    out = view(taps[0], values[0])
+    for t, v in zip(taps[1:], values[1:]):
+        with np.errstate(invalid="ignore", over="ignore"):
+            out = (out + view(t, v)).astype(F)
     return out
 
 
@@ -414,7 +572,9 @@ def symmetric_compute_eigenvalues(S_elems, sort="descending", abs_sort=False):
     eigs = np.linalg.eigvalsh(mat)
     eigs = np.moveaxis(eigs, -1, 0)
     if abs_sort:
-        eigs = np.take_along_axis(eigs, np.abs(eigs).argsort(0), 0)
+        # CuPy's argsort is a stable (radix) sort: equal magnitudes keep the
+        # ascending order of eigvalsh
+        eigs = np.take_along_axis(eigs, np.abs(eigs).argsort(0, kind="stable"), 0)
     if sort == "descending":
         eigs = eigs[::-1]
     return eigs

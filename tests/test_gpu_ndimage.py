@@ -359,3 +359,84 @@ def test_register_blocked_uniform(gpu, size, origin, mode):
     x = _img((70, 90), np.float32, seed=6)
     exp = on.uniform_filter(x, size, mode=mode, cval=2.0, origin=origin, acc="emulate")
     _exact(_run(ndi.uniform_filter, x, size, mode=mode, cval=2.0, origin=origin), exp)
+
+
+@pytest.mark.parametrize("fn,kw", [
+    ("laplace", {}),
+    ("gaussian_laplace", dict(sigma=1.3)),
+    ("gaussian_laplace", dict(sigma=(0.8, 2.0), axes=(0, -1))),
+    ("gaussian_gradient_magnitude", dict(sigma=1.1)),
+    ("gaussian_gradient_magnitude", dict(sigma=2.0, axes=(1,))),
+])
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("shape", [(64, 57), (7, 9, 11)])
+def test_laplace_family(gpu, fn, kw, mode, shape):
+    """laplace / gaussian_laplace / gaussian_gradient_magnitude
+    (_ndimage_filters.py:806-1158): float32 bit-exact to the oracle's
+    emulation (per-axis passes, then adds / squares rounded in float32);
+    float64 within 1e-12 relative."""
+    ndi = _ndi()
+    if len(shape) == 3 and isinstance(kw.get("sigma"), tuple):
+        pytest.skip("two sigmas for two of three axes")
+    x = _img(shape, np.float32)
+    exp = getattr(on, fn)(x, mode=mode, cval=0.25, acc="emulate", **kw)
+    for algo in (None, "elementwise"):
+        _exact(_run(getattr(ndi, fn), x, mode=mode, cval=0.25, algorithm=algo, **kw), exp)
+    x64 = _img(shape, np.float64)
+    exp64 = getattr(on, fn)(x64, mode=mode, cval=0.25, **kw)
+    got64 = _run(getattr(ndi, fn), x64, mode=mode, cval=0.25, **kw)
+    np.testing.assert_allclose(got64, exp64, rtol=1e-12, atol=1e-12 * np.abs(exp64).max())
+
+
+@pytest.mark.parametrize("out_dtype", [None, np.int16, np.int32, np.float32])
+def test_laplace_integer_input_and_output(gpu, out_dtype):
+    """uint8 input: the output defaults to uint8 (wrapping adds, saturating
+    float -> integer conversion of each axis's pass), as in the reference."""
+    ndi = _ndi()
+    x = np.random.default_rng(31).integers(0, 256, (40, 33)).astype(np.uint8)
+    _exact(_run(ndi.laplace, x, output=out_dtype),
+           on.laplace(x, output=out_dtype, acc="emulate"))
+    _exact(_run(ndi.gaussian_gradient_magnitude, x, 1.5, output=out_dtype),
+           on.gaussian_gradient_magnitude(x, 1.5, output=out_dtype, acc="emulate"))
+
+
+def test_laplace_family_preallocated_strided_output(gpu):
+    """A preallocated non-contiguous output receives the same values."""
+    ndi = _ndi()
+    x = _img((30, 40), np.float32)
+    exp = on.gaussian_laplace(x, 1.2, acc="emulate")
+    big = torch.zeros((30, 80), dtype=torch.float32, device="cuda")
+    out = big[:, ::2]
+    r = ndi.gaussian_laplace(torch.from_numpy(x).cuda(), 1.2, output=out)
+    assert r is out
+    _exact(out.cpu().numpy(), exp)
+    exp2 = on.gaussian_gradient_magnitude(x, 1.2, acc="emulate")
+    ndi.gaussian_gradient_magnitude(torch.from_numpy(x).cuda(), 1.2, output=out)
+    _exact(out.cpu().numpy(), exp2)
+
+
+@pytest.mark.parametrize("dt", [np.uint8, np.int8, np.int16, np.uint16, np.int32, np.int64])
+def test_sqrt_and_square_acc_integers(gpu, dt):
+    """The integer element kernels: squares and sums wrap modulo 2^bits, the
+    in-place sqrt truncates (negative -> 0), against the oracle's numpy
+    restatement over every value of the 8/16-bit types."""
+    from cucim_amd import _array, _lib
+
+    info = np.iinfo(dt)
+    if info.bits <= 16:
+        a = np.arange(info.min, info.max + 1, dtype=np.int64).astype(dt)
+    else:
+        a = np.random.default_rng(9).integers(info.min // 2, info.max // 2, 70000).astype(dt)
+    b = np.roll(a, 7)
+    ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    lib = _lib.lib()
+    _lib.check(lib.hc_square_acc(_array.hc_array(ta), _array.hc_array(ta), 1,
+                                 _array.stream()), "sq")
+    _lib.check(lib.hc_square_acc(_array.hc_array(ta), _array.hc_array(tb), 0,
+                                 _array.stream()), "sq")
+    exp = on._wrap_or_round(on._wrap_or_round(a, a, "mul"), on._wrap_or_round(b, b, "mul"),
+                            "add")
+    _exact(ta.cpu().numpy(), exp)
+    tc = torch.from_numpy(a).cuda()
+    _lib.check(lib.hc_sqrt_inplace(_array.hc_array(tc), _array.stream()), "sqrt")
+    _exact(tc.cpu().numpy(), on._sqrt_unsafe(a))

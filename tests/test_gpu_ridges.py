@@ -145,3 +145,14 @@ def test_output_dtype(gpu, name, dtype):
     x = torch.rand((20, 24), device="cuda").to(dtype)
     exp = torch.float64 if dtype == torch.float64 else torch.float32
     assert _f(name)(x, sigmas=(1,)).dtype == exp
+
+
+@pytest.mark.parametrize("name", ["sato", "meijering", "frangi"])
+def test_sigmas_from_generator(gpu, name):
+    """sigmas may be any iterable of floats, a one-shot generator included
+    (filters/ridges.py:148-225 iterates it once)."""
+    x = _vol((40, 36), 14)
+    a = _np(_f(name)(_t(x), sigmas=(s for s in (1.0, 2.0)), mode="reflect"))
+    b = _np(_f(name)(_t(x), sigmas=[1.0, 2.0], mode="reflect"))
+    np.testing.assert_array_equal(a, b)
+    assert np.abs(b).max() > 0

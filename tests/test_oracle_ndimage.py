@@ -184,3 +184,85 @@ def test_emulated_accumulation_is_float32_sequential():
             j = min(max(i + k - 2, 0), 39)
             s = on.fma_f32(x[j], w[k], s)
         assert y[i] == s
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("axes", [None, (0, 2), (1,)])
+def test_laplace_family_vs_scipy(mode, axes):
+    """laplace / gaussian_laplace / gaussian_gradient_magnitude
+    (_ndimage_filters.py:806-1158) against scipy.ndimage's own (an
+    independent implementation of the same per-axis definitions)."""
+    x = np.random.default_rng(5).standard_normal((14, 17, 11))
+    kw = dict(mode=mode, cval=0.5, axes=axes)
+    np.testing.assert_allclose(on.laplace(x, **kw), sndi.laplace(x, **kw), atol=1e-13)
+    np.testing.assert_allclose(on.gaussian_laplace(x, 1.3, **kw),
+                               sndi.gaussian_laplace(x, 1.3, **kw), atol=1e-13)
+    np.testing.assert_allclose(on.gaussian_gradient_magnitude(x, 0.9, **kw),
+                               sndi.gaussian_gradient_magnitude(x, 0.9, **kw), atol=1e-13)
+
+
+def test_laplace_family_closed_forms():
+    """A quadratic has Laplacian 2*sum(a) away from the border; the gradient
+    magnitude of a plane is the norm of its slope.  With the kernels
+    truncated at 12 sigma, the discrete Gaussian moments equal the continuous
+    ones up to the aliasing term exp(-2 pi^2 sigma^2) (< 1e-19 at sigma 1.5)."""
+    z, y = np.mgrid[0:60, 0:70].astype(np.float64)
+    q = 0.5 * z * z + 1.5 * y * y
+    lap = on.laplace(q)
+    np.testing.assert_allclose(lap[2:-2, 2:-2], 4.0, atol=1e-9)
+    plane = 0.3 * z - 0.4 * y
+    g = on.gaussian_gradient_magnitude(plane, 1.5, truncate=12.0)
+    np.testing.assert_allclose(g[19:-19, 19:-19], 0.5, rtol=1e-9)
+    gl = on.gaussian_laplace(q, 1.5, truncate=12.0)
+    np.testing.assert_allclose(gl[19:-19, 19:-19], 4.0, rtol=1e-9)
+
+
+def test_laplace_integer_output_vs_scipy():
+    """Integer outputs: the per-axis results are cast to the output dtype and
+    added with wrap-around.  The per-axis values are integers of magnitude
+    < 128 here, where the reference's float -> unsigned cast (negatives
+    wrap) and scipy's C cast agree; beyond the destination range the device
+    conversion saturates, which scipy does not model."""
+    x = np.random.default_rng(6).integers(0, 60, (30, 41)).astype(np.uint8)
+    for dt in (np.uint8, np.int16, np.int32):
+        np.testing.assert_array_equal(on.laplace(x, output=dt), sndi.laplace(x, output=dt))
+    # gradient magnitude with integer outputs (squares and sums in the output
+    # dtype, then the unsafe in-place sqrt): Sobel derivatives are integers,
+    # in range for signed 32/64-bit outputs (unsigned ones would wrap the
+    # negative first-pass values, where scipy's cast and the device's part)
+    y = np.random.default_rng(7).integers(0, 30, (25, 26)).astype(np.uint16)
+    for dt in (np.int32, np.int64):
+        got = on.generic_gradient_magnitude_arrays(
+            [on.sobel(y, a, output=dt) for a in range(2)])
+        ref = sndi.generic_gradient_magnitude(y.astype(dt), sndi.sobel, output=dt)
+        np.testing.assert_array_equal(got, ref)
+    np.testing.assert_allclose(on.gaussian_gradient_magnitude(y, 1.0, output=np.float64),
+                               sndi.gaussian_gradient_magnitude(y, 1.0, output=np.float64),
+                               rtol=1e-13)
+
+
+@pytest.mark.parametrize("dt", [np.uint8, np.int8, np.uint16, np.int16, np.uint32, np.int32])
+def test_sqrt_unsafe_exhaustive(dt):
+    """The unsafe in-place sqrt of integers equals numpy's own
+    sqrt(a, a, casting='unsafe') for every non-negative value (all values of
+    the 8/16-bit types; a sample of the wider ones), and maps negatives
+    (NaN) to 0."""
+    info = np.iinfo(dt)
+    if info.bits <= 16:
+        a = np.arange(info.min, info.max + 1, dtype=np.int64).astype(dt)
+    else:
+        r = np.random.default_rng(8)
+        a = np.concatenate([r.integers(info.min, info.max, 100000, dtype=np.int64),
+                            np.arange(0, 70000), np.array([info.max])]).astype(dt)
+    got = on._sqrt_unsafe(a)
+    nonneg = a >= 0
+    ref = a.copy()
+    with np.errstate(invalid="ignore"):
+        np.sqrt(ref, out=ref, casting="unsafe")
+    np.testing.assert_array_equal(got[nonneg], ref[nonneg])
+    assert np.all(got[~nonneg] == 0)
+    # floor(sqrt(v)) exactly: the float16 step of the 8-bit types truncates
+    # to the integer square root
+    v = a[nonneg].astype(np.int64)
+    isq = np.floor(np.sqrt(v.astype(np.float64))).astype(np.int64)
+    np.testing.assert_array_equal(got[nonneg].astype(np.int64), isq)
