@@ -21,7 +21,8 @@ OBJ = os.path.join(ROOT, "_build")
 
 SOURCES = ["hc_host.hip", "k_correlate.hip", "k_hessian.hip", "k_pointwise.hip",
            "k_stencil2d.hip", "k_ridges.hip", "k_cnn3d.hip",
-           "k_frangi_check.hip", "k_eig_nd.hip"]
+           "k_frangi_check.hip", "k_eig_nd.hip", "k_measure.hip",
+           "hc_taps.hip"]
 
 ARCH = "gfx950"
 # Packed FP32 instructions are disabled for every HIP translation unit.

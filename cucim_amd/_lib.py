@@ -58,6 +58,15 @@ ENTRY_POINTS = (
     "hc_binary",
     "hc_div_scalar",
     "hc_correlate_sparse",
+    "hc_gaussian_kernel1d",
+    "hc_gaussian_nd",
+    "hc_hessian_matrix",
+    "hc_frangi_sigmas",
+    "hc_minmax",
+    "hc_histogram",
+    "hc_li_means",
+    "hc_label_work_elems",
+    "hc_label",
     "hc_sqrt_inplace",
     "hc_copy_cast",
     "hc_batch2d_plan",
@@ -123,9 +132,9 @@ _SIGNATURES = {
     "hc_correlate_nd": ([_A, _A, _v, _P(ctypes.c_int64), _P(ctypes.c_int32),
                          _P(ctypes.c_int32), _i, _d, _v], _i),
     "hc_separable2d": ([_A, _A, _T, _T, _i, _i, _d, _v], _i),
-    "hc_hessian_gaussian": ([_A, _A, _i, _T, _T, _i, _d, _i, _v], _i),
+    "hc_hessian_gaussian": ([_A, _A, _i, _T, _T, _i, _d, _i, _v, _v], _i),
     "hc_sym_eigvals": ([_A, _i, _A, _i, _i, _v], _i),
-    "hc_frangi": ([_A, _A, _T, _T, _i, _d, _d, _d, _i, _i, _i, _d, _A, _i, _v, _v], _i),
+    "hc_frangi": ([_A, _A, _T, _T, _i, _d, _d, _d, _i, _i, _i, _d, _A, _i, _v, _v, _v], _i),
     "hc_frangi_from_eigvals": ([_A, _i, _A, _d, _d, _d, _i, _i, _v, _v], _i),
     "hc_rgb2gray": ([_A, _A, _v], _i),
     "hc_to_float": ([_A, _A, _v], _i),
@@ -137,6 +146,15 @@ _SIGNATURES = {
     "hc_binary": ([_A, _A, _A, _i, _v], _i),
     "hc_div_scalar": ([_A, _d, _v], _i),
     "hc_correlate_sparse": ([_A, _A, _v, _v, _v, _i, _i, _i, _v], _i),
+    "hc_gaussian_kernel1d": ([_d, _i, ctypes.c_int64, _v], _i),
+    "hc_gaussian_nd": ([_A, _A, _v, _v, _v, _v, _d, _d, _A, _v], _i),
+    "hc_hessian_matrix": ([_A, _A, _i, _v, _i, _d, _i, _v, _v], _i),
+    "hc_frangi_sigmas": ([_A, _A, _v, _i, _d, _d, _d, _i, _i, _i, _d, _A, _i, _v, _v], _i),
+    "hc_minmax": ([_A, _v, _v, ctypes.c_int64, _v], _i),
+    "hc_histogram": ([_A, ctypes.c_int64, _v, ctypes.c_int64, _v, _v], _i),
+    "hc_li_means": ([_A, _d, _d, _v, _v, ctypes.c_int64, _v], _i),
+    "hc_label_work_elems": ([ctypes.c_int64], ctypes.c_int64),
+    "hc_label": ([_A, ctypes.c_int64, _i, _A, _v, ctypes.c_int64, _v, _v], _i),
     "hc_sqrt_inplace": ([_A, _v], _i),
     "hc_copy_cast": ([_A, _A, _v], _i),
     "hc_batch2d_plan": ([_v, _i, _i, _i, _v, ctypes.c_int64], ctypes.c_int64),

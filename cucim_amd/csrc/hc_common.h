@@ -127,7 +127,12 @@ __device__ __forceinline__ bool finite(W v) {
 template <typename W, bool TYPED, class Map>
 __device__ __forceinline__ bool stage_lds(W* __restrict__ lds, int total, int tid, int nthreads,
                                           const void* __restrict__ in, int dt, W cval,
-                                          bool negate, Map map) {
+                                          bool negate, Map map,
+                                          const double* __restrict__ center = nullptr) {
+  // center (device, optional): the value subtracted from every element
+  // (after the negation, as c - x; the constant padding is not shifted)
+  const bool ctr = center != nullptr;
+  const W c = ctr ? (W)*center : (W)0;
   bool bad = false;
   for (int e0 = tid; e0 < total; e0 += nthreads * kStageBatch) {
     W v[kStageBatch];
@@ -141,7 +146,7 @@ __device__ __forceinline__ bool stage_lds(W* __restrict__ lds, int total, int ti
         const int64_t g = map(e, slot[u]);
         if (g >= 0) {
           const W x = TYPED ? load_w<W>(in, g) : load_as<W>(in, g, dt);
-          v[u] = negate ? -x : x;
+          v[u] = ctr ? (negate ? c - x : x - c) : (negate ? -x : x);
         }
       }
     }
@@ -158,10 +163,11 @@ __device__ __forceinline__ bool stage_lds(W* __restrict__ lds, int total, int ti
 template <typename W, class Map>
 __device__ __forceinline__ bool stage_any(W* __restrict__ lds, int total, int tid, int nthreads,
                                           const void* __restrict__ in, int dt, W cval,
-                                          bool negate, Map map) {
+                                          bool negate, Map map,
+                                          const double* __restrict__ center = nullptr) {
   if (dt == w_dtype<W>())
-    return stage_lds<W, true>(lds, total, tid, nthreads, in, dt, cval, negate, map);
-  return stage_lds<W, false>(lds, total, tid, nthreads, in, dt, cval, negate, map);
+    return stage_lds<W, true>(lds, total, tid, nthreads, in, dt, cval, negate, map, center);
+  return stage_lds<W, false>(lds, total, tid, nthreads, in, dt, cval, negate, map, center);
 }
 
 // Saturating truncation toward zero (float -> integer conversion of the GPU;

@@ -58,6 +58,10 @@ struct ChainArgs {
   int rows;     // contiguous variant: lines per work-group
   int mode;
   int negate;   // first pass of Frangi with black_ridges=False: image = -image
+  // first pass, optional (device): a constant subtracted from the image (see
+  // image_center); every Hessian element is unchanged by it in exact
+  // arithmetic, the staged values are smaller
+  const double* center;
   W cval;
   TapPack<W> S, D;  // same padded length and anchor
   int t0, Km;       // matrix-core variant: taps t0 .. t0 + Km - 1 hold every
@@ -107,7 +111,8 @@ __global__ __launch_bounds__(256) void k_chain_strided(const ChainArgs<W> a) {
                          if (r < 0) return -1;
                          const int64_t lo = L / inner, li = L - lo * inner;
                          return lo * N * inner + li + r * inner;
-                       });
+                       },
+                       a.center);
   }
   const bool skip0 = __syncthreads_or(bad);
   // 2. first passes over positions [0, Nr) into registers (<= NI items/lane)
@@ -349,10 +354,11 @@ __device__ __forceinline__ void chain_pass(bool exact, f32x4_t (&acc)[NS][NBW], 
     chain_pass_mfma<NBW, NS, NW>(acc, X, wp, nd, nblk, wv, lane, Km);
 }
 
-__device__ __forceinline__ float load_in(const void* in, int dt, int64_t g, bool negate) {
+__device__ __forceinline__ float load_in(const void* in, int dt, int64_t g, bool negate,
+                                         bool ctr, float c) {
   const float y =
       dt == HC_FLOAT32 ? reinterpret_cast<const float*>(in)[g] : load_as<float>(in, g, dt);
-  return negate ? -y : y;
+  return ctr ? (negate ? c - y : y - c) : (negate ? -y : y);
 }
 
 // Zero-padded trimmed taps of both sets into LDS.
@@ -433,6 +439,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(5))) vo
       // so the loads of a lane would run one after another.
       const float* in = reinterpret_cast<const float*>(jb.in);
       const bool neg = a.negate != 0;
+      const bool ctr = a.center != nullptr;
+      const float c = ctr ? (float)*a.center : 0.f;
 #pragma unroll
       for (int u = 0; u < NPF; ++u) {
         const int n = ns0 + 4 * NW * u;
@@ -443,13 +451,17 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(5))) vo
       for (int u = 0; u < NPF; ++u) {
         const int n = ns0 + 4 * NW * u;
         const bool use = ok && n < N;
-        v[u] = use ? (neg ? -v[u] : v[u]) : 0.f;
+        const float y = v[u];
+        v[u] = use ? (ctr ? (neg ? c - y : y - c) : (neg ? -y : y)) : 0.f;
       }
     } else {
 #pragma unroll
       for (int u = 0; u < NPF; ++u) {
         const int n = ns0 + 4 * NW * u;
-        v[u] = (ok && n < N) ? load_in(jb.in, jb.in_dt, base + n * inner, a.negate != 0) : 0.f;
+        v[u] = (ok && n < N) ? load_in(jb.in, jb.in_dt, base + n * inner, a.negate != 0,
+                                       a.center != nullptr,
+                                       a.center ? (float)*a.center : 0.f)
+                             : 0.f;
       }
     }
   };
@@ -607,7 +619,8 @@ __global__ __launch_bounds__(256) void k_chain_contig(const ChainArgs<W> a) {
                          if (row >= nlines) return -1;
                          const int64_t r = remap((int64_t)q - anc, N, mode);
                          return r < 0 ? -1 : row * N + r;
-                       });
+                       },
+                       a.center);
   }
   const bool skip0 = __syncthreads_or(bad);
   const int nq = Nr / 4;
@@ -1288,7 +1301,8 @@ inline int launch_chain_mfma(ChainArgs<float>& a, hipStream_t st) {
 
 template <typename W>
 int run_chain_pass(const ChainJob* jobs, int njobs, const hc_array* shape_src, int axis,
-                   const AxisTaps<W>& tp, int mode, double cval, int negate, hipStream_t st) {
+                   const AxisTaps<W>& tp, int mode, double cval, int negate, hipStream_t st,
+                   const double* center = nullptr) {
   ChainArgs<W> a;
   std::memset(&a, 0, sizeof(a));
   for (int j = 0; j < njobs; ++j) a.job[j] = jobs[j];
@@ -1297,6 +1311,7 @@ int run_chain_pass(const ChainJob* jobs, int njobs, const hc_array* shape_src, i
   a.inner = ln.inner;
   a.mode = mode;
   a.negate = negate;
+  a.center = center;
   a.cval = (W)cval;
   a.S = tp.S;
   a.D = tp.D;
@@ -1438,12 +1453,115 @@ int plan_fits(const hc_array* img, const AxisTaps<W>* tp) {
   return HC_OK;
 }
 
+// ------------------------------------------------------ image centering --
+// Every Hessian element is a composite of two first-derivative kernels along
+// its axes (antisymmetric, zero sum), so it annihilates constants: H(I - c)
+// = H(I) exactly, for every mode that extends the image by its own values
+// (not 'constant').  The first pass stages I - c with c the image's
+// midrange, which halves the magnitude of the staged values (smaller
+// rounding in every later pass) and makes an image and its 255-complement
+// (black vs white ridges of 8-bit data) stage identical values.  c is
+// reduced on the device into the scratch (bytes 32..63), so no host
+// synchronisation is needed; with NaN or infinite values c = 0.
+__device__ __forceinline__ unsigned long long order_key(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double key_value(unsigned long long k) {
+  return __longlong_as_double((long long)((k >> 63) ? (k & 0x7fffffffffffffffull) : ~k));
+}
+
+__global__ void k_center_reset(unsigned long long* s) {
+  if (threadIdx.x == 0) {
+    s[0] = ~0ull;  // min key
+    s[1] = 0ull;   // max key
+    s[2] = 0ull;   // non-finite seen
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_center_reduce(const void* __restrict__ x, int dt,
+                                                       int64_t n, unsigned long long* s) {
+  __shared__ double slo[256], shi[256];
+  __shared__ int sbad;
+  if (threadIdx.x == 0) sbad = 0;
+  __syncthreads();
+  double lo = INFINITY, hi = -INFINITY;
+  bool bad = false;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    double v;
+    if constexpr (std::is_same<T, void>::value)
+      v = load_as<double>(x, i, dt);
+    else
+      v = (double)reinterpret_cast<const T*>(x)[i];
+    if (!isfinite(v)) {
+      bad = true;
+    } else {
+      lo = v < lo ? v : lo;
+      hi = v > hi ? v : hi;
+    }
+  }
+  if (bad) sbad = 1;
+  const int tid = threadIdx.x;
+  slo[tid] = lo;
+  shi[tid] = hi;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tid < o) {
+      slo[tid] = slo[tid + o] < slo[tid] ? slo[tid + o] : slo[tid];
+      shi[tid] = shi[tid + o] > shi[tid] ? shi[tid + o] : shi[tid];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (slo[0] <= shi[0]) {
+      atomicMin(s, order_key(slo[0]));
+      atomicMax(s + 1, order_key(shi[0]));
+    }
+    if (sbad) atomicOr(s + 2, 1ull);
+  }
+}
+
+__global__ void k_center_final(const unsigned long long* s, double* c) {
+  if (threadIdx.x == 0) {
+    const double lo = key_value(s[0]), hi = key_value(s[1]);
+    c[0] = (s[2] || s[0] == ~0ull || !(lo <= hi)) ? 0.0 : 0.5 * (lo + hi);
+  }
+}
+
+// *center = device address of the midrange of `in` (kernels enqueued on st),
+// or nullptr when centering does not apply (no scratch, constant mode).
+int image_center(const hc_array* in, int mode, void* scratch, hipStream_t st,
+                 const double** center) {
+  *center = nullptr;
+  if (!scratch || mode == HC_MODE_CONSTANT) return HC_OK;
+  const int64_t n = numel(in);
+  if (n == 0) return HC_OK;
+  unsigned long long* s =
+      reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(scratch) + 32);
+  double* c = reinterpret_cast<double*>(reinterpret_cast<char*>(scratch) + 56);
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 4095) / 4096));
+  HC_LAUNCH("k_center_reset", st, k_center_reset, dim3(1), dim3(64), 0, s);
+  if (in->dtype == HC_FLOAT32)
+    HC_LAUNCH("k_center_reduce", st, k_center_reduce<float>, dim3(grid), dim3(256), 0, in->data, in->dtype, n, s);
+  else if (in->dtype == HC_FLOAT64)
+    HC_LAUNCH("k_center_reduce", st, k_center_reduce<double>, dim3(grid), dim3(256), 0, in->data, in->dtype, n, s);
+  else
+    HC_LAUNCH("k_center_reduce", st, k_center_reduce<void>, dim3(grid), dim3(256), 0, in->data, in->dtype, n, s);
+  HC_LAUNCH("k_center_final", st, k_center_final, dim3(1), dim3(64), 0, s, c);
+  HC_HIP(hipGetLastError());
+  *center = c;
+  return HC_OK;
+}
+
 // Runs the non-final passes of the Hessian schedule from `img` into the
 // buffers buf[k] (one per output pair).  After it, buf[k] holds output k's
 // chains along every axis but the last.
 template <typename W>
 int hessian_prefix(const hc_array* img, W* const* buf, const Pair* P, int nP,
-                   const AxisTaps<W>* tp, int mode, double cval, int negate, hipStream_t st) {
+                   const AxisTaps<W>* tp, int mode, double cval, int negate, hipStream_t st,
+                   const double* center) {
   const int nd = img->ndim;
   // pass over axis 0 from the image: distinct chains
   int node_of[6];
@@ -1470,7 +1588,7 @@ int hessian_prefix(const hc_array* img, W* const* buf, const Pair* P, int nP,
   job0.nout = nnode;
   for (int m = 0; m < nnode; ++m)
     job0.o[m] = ChainOut{buf[node_buf[m]], node_first[m], node_second[m]};
-  int rc = run_chain_pass<W>(&job0, 1, img, 0, tp[0], mode, cval, negate, st);
+  int rc = run_chain_pass<W>(&job0, 1, img, 0, tp[0], mode, cval, negate, st, center);
   if (rc != HC_OK) return rc;
   if (nd == 3) {
     // pass over axis 1: one job per axis-0 node
@@ -1494,7 +1612,8 @@ int hessian_prefix(const hc_array* img, W* const* buf, const Pair* P, int nP,
 
 template <typename W>
 int hessian_impl(const hc_array* in, const hc_array* H, int nH, const hc_taps* ts,
-                 const hc_taps* td, int mode, double cval, int order_xy, hipStream_t st) {
+                 const hc_taps* td, int mode, double cval, int order_xy, void* scratch,
+                 hipStream_t st) {
   const int nd = in->ndim;
   AxisTaps<W> tp[3];
   for (int x = 0; x < nd; ++x) {
@@ -1509,7 +1628,10 @@ int hessian_impl(const hc_array* in, const hc_array* H, int nH, const hc_taps* t
   W* buf[6];
   for (int k = 0; k < nH; ++k) buf[k] = reinterpret_cast<W*>(H[k].data);
   if (numel(in) == 0) return HC_OK;
-  rc = hessian_prefix<W>(in, buf, P, nP, tp, mode, cval, 0, st);
+  const double* center = nullptr;
+  rc = image_center(in, mode, scratch, st, &center);
+  if (rc != HC_OK) return rc;
+  rc = hessian_prefix<W>(in, buf, P, nP, tp, mode, cval, 0, st, center);
   if (rc != HC_OK) return rc;
   FinalArgs<W> a;
   std::memset(&a, 0, sizeof(a));
@@ -1530,7 +1652,8 @@ template <typename W>
 int frangi_impl(const hc_array* in, const hc_array* out, const hc_taps* ts, const hc_taps* td,
                 int nsig, double alpha, double beta, double gamma, bool gamma_known,
                 int black_ridges, int mode,
-                double cval, const hc_array* work, int nwork, void* scratch, hipStream_t st) {
+                double cval, const hc_array* work, int nwork, void* scratch,
+                const double* center_in, hipStream_t st) {
   const int nd = in->ndim;
   const int nH = nd == 3 ? 6 : 3;
   if (nwork < nH) return fail(HC_ERR_VALUE, "not enough work arrays");
@@ -1562,9 +1685,16 @@ int frangi_impl(const hc_array* in, const hc_array* out, const hc_taps* ts, cons
   for (int k = 0; k < nH; ++k) buf[k] = reinterpret_cast<W*>(work[k].data);
   W* fo = reinterpret_cast<W*>(out->data);
   const int negate = black_ridges ? 0 : 1;
+  const double* center = nullptr;
+  if (center_in) {
+    if (mode != HC_MODE_CONSTANT) center = center_in;
+  } else {
+    const int rc = image_center(in, mode, scratch, st, &center);
+    if (rc != HC_OK) return rc;
+  }
   for (int s = 0; s < nsig; ++s) {
     const AxisTaps<W>* tp = &tps[(size_t)s * nd];
-    int rc = hessian_prefix<W>(in, buf, P, nH, tp, mode, cval, negate, st);
+    int rc = hessian_prefix<W>(in, buf, P, nH, tp, mode, cval, negate, st, center);
     if (rc != HC_OK) return rc;
     FinalArgs<W> a;
     std::memset(&a, 0, sizeof(a));
@@ -1676,7 +1806,7 @@ int hc_max_resident_line(int w_f64) {
 
 int hc_hessian_gaussian(const hc_array* in, const hc_array* H, int nH, const hc_taps* taps_s,
                         const hc_taps* taps_d, int mode, double cval, int order_xy,
-                        void* stream) {
+                        void* scratch, void* stream) {
   int rc = check_float_image(in);
   if (rc != HC_OK) return rc;
   if (in->dtype != HC_FLOAT32 && in->dtype != HC_FLOAT64)
@@ -1696,8 +1826,8 @@ int hc_hessian_gaussian(const hc_array* in, const hc_array* H, int nH, const hc_
     if ((taps_s[x].w_f64 != 0) != f64 || (taps_d[x].w_f64 != 0) != f64)
       return fail(HC_ERR_VALUE, "weights dtype must match the image dtype");
   hipStream_t st = (hipStream_t)stream;
-  return f64 ? hessian_impl<double>(in, H, nH, taps_s, taps_d, mode, cval, order_xy, st)
-             : hessian_impl<float>(in, H, nH, taps_s, taps_d, mode, cval, order_xy, st);
+  return f64 ? hessian_impl<double>(in, H, nH, taps_s, taps_d, mode, cval, order_xy, scratch, st)
+             : hessian_impl<float>(in, H, nH, taps_s, taps_d, mode, cval, order_xy, scratch, st);
 }
 
 int hc_sym_eigvals(const hc_array* S, int nS, const hc_array* eig, int sort, int abs_sort,
@@ -1756,7 +1886,7 @@ int hc_frangi_from_eigvals(const hc_array* eig, int ndim, const hc_array* out, d
 int hc_frangi(const hc_array* in, const hc_array* out, const hc_taps* taps_s,
               const hc_taps* taps_d, int nsig, double alpha, double beta, double gamma,
               int has_gamma, int black_ridges, int mode, double cval, const hc_array* work,
-              int nwork, void* scratch, void* stream) {
+              int nwork, void* scratch, const double* center, void* stream) {
   int rc = check_float_image(in);
   if (rc != HC_OK) return rc;
   if (!out || !same_shape(in, out) || !is_c_contiguous(out))
@@ -1776,10 +1906,10 @@ int hc_frangi(const hc_array* in, const hc_array* out, const hc_taps* taps_s,
   hipStream_t st = (hipStream_t)stream;
   return f64 ? frangi_impl<double>(in, out, taps_s, taps_d, nsig, alpha, beta, gamma,
                                    has_gamma != 0, black_ridges, mode, cval, work, nwork,
-                                   scratch, st)
+                                   scratch, center, st)
              : frangi_impl<float>(in, out, taps_s, taps_d, nsig, alpha, beta, gamma,
                                   has_gamma != 0, black_ridges, mode, cval, work, nwork,
-                                  scratch, st);
+                                  scratch, center, st);
 }
 
 }  // extern "C"

@@ -1,3 +1,3 @@
 """Drop-in subset of ``cucim.skimage`` on hand-written gfx950 kernels."""
 
-from . import color, feature, filters  # noqa: F401
+from . import color, exposure, feature, filters, measure  # noqa: F401

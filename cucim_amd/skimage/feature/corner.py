@@ -6,6 +6,7 @@ from itertools import combinations_with_replacement
 from warnings import warn
 
 import numpy as np
+import torch
 
 from ... import _array, _lib
 from .._shared.utils import _supported_float_type, astype
@@ -94,9 +95,10 @@ def _hessian_matrix_with_gaussian(image, sigma=1, mode="reflect", cval=0, order=
         H = [_array.empty(x.shape, float_dtype) for _ in range(nH)]
         f64 = float_dtype == np.float64
         ts, td, keep = _taps_arrays(hessian_kernels(sigma, ndim, float_dtype), f64)
+        scratch = torch.empty(64, dtype=torch.uint8, device=x.device)
         rc = _lib.lib().hc_hessian_gaussian(
             _array.hc_array(x), _array.hc_arrays(H), nH, ts, td, _mode_code(mode),
-            float(cval), 1 if order == "xy" else 0, _array.stream())
+            float(cval), 1 if order == "xy" else 0, scratch.data_ptr(), _array.stream())
         if rc == _lib.OK:
             return H
         if rc != _lib.ERR_RESOURCE:
@@ -150,7 +152,8 @@ def _symmetric_compute_eigenvalues(S_elems, sort="descending", abs_sort=False):
     (Jacobi rotations on the device) for other orders up to 8; returned as
     one (ndim,) + shape array."""
     S = [_array.as_tensor(s) for s in S_elems]
-    if len(S) not in (3, 6):
+    ne = {3: 2, 6: 3}.get(len(S))
+    if ne is None:
         # the eigvalsh branch builds matrices of the image's rank from the
         # first n(n+1)/2 elements (_symmetric_image, corner.py:758-783)
         ne = S[0].dim()

@@ -22,3 +22,4 @@ from .edges import (  # noqa: F401
     sobel_v,
 )
 from .ridges import frangi, hessian, meijering, sato  # noqa: F401
+from .thresholding import threshold_li, threshold_otsu, threshold_yen  # noqa: F401

@@ -44,14 +44,18 @@ class FrangiPlan:
             kern += hessian_kernels(s, self.ndim, self.dtype)
         self.ts, self.td, self._keep = _taps_arrays(kern, self.dtype == np.float64)
 
-    def run(self, image, out, alpha=0.5, beta=0.5, gamma=None, black_ridges=True, cval=0):
-        """Fused schedule; returns an hc status code."""
+    def run(self, image, out, alpha=0.5, beta=0.5, gamma=None, black_ridges=True, cval=0,
+            center=None):
+        """Fused schedule; returns an hc status code.  center: an optional
+        one-element float64 device tensor subtracted from the image before
+        the first pass instead of the image's own midrange."""
         x = image.contiguous()
         return _lib.lib().hc_frangi(
             _array.hc_array(x), _array.hc_array(out), self.ts, self.td, len(self.sigmas),
             float(alpha), float(beta), 0.0 if gamma is None else float(gamma),
-            0 if gamma is None else 1, 1 if black_ridges else 0, _mode_code(self.mode), float(cval), self.work_hc,
-            len(self.work), self.scratch.data_ptr(), _array.stream())
+            0 if gamma is None else 1, 1 if black_ridges else 0, _mode_code(self.mode),
+            float(cval), self.work_hc, len(self.work), self.scratch.data_ptr(),
+            None if center is None else center.data_ptr(), _array.stream())
 
 
 def _frangi_sequential(image, sigmas, alpha, beta, gamma, black_ridges, mode, cval, out):

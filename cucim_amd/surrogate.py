@@ -460,7 +460,7 @@ class CheckedSurrogate:
             rel = np.where((n > 0) & (den == 0), np.where(dn > 0, np.inf, 0.0), rel)
         return rel
 
-    def _plain(self, region):
+    def _plain(self, region, center=None):
         from .skimage.filters.ridges import FrangiPlan
 
         shp = tuple(int(s) for s in region.shape)
@@ -470,8 +470,20 @@ class CheckedSurrogate:
             self._plans[shp] = plan
         out = torch.empty_like(region)
         _check(plan.run(region, out, alpha=self.alpha, beta=self.beta,
-                        gamma=self.model.gamma, black_ridges=False), "plain frangi")
+                        gamma=self.model.gamma, black_ridges=False, center=center),
+               "plain frangi")
         return out
+
+    @staticmethod
+    def _center(vol):
+        """The midrange the plain filter subtracts from the whole volume
+        (0 with non-finite values), as a device scalar: recompute regions use
+        it so that they reproduce the whole volume's values bit for bit."""
+        from .skimage.exposure.exposure import _range
+
+        lo, hi = _range(vol.reshape(-1))
+        c = 0.5 * (lo + hi) if (np.isfinite(lo) and np.isfinite(hi)) else 0.0
+        return torch.tensor([c], dtype=torch.float64, device=vol.device)
 
     def _region(self, i):
         """Brick i's bounds and those of its recompute region (the brick plus
@@ -503,6 +515,7 @@ class CheckedSurrogate:
                          for _, (rz0, rz1, ry0, ry1, rx0, rx1) in regions)
         mass = region_vox > D * H * W
         full = self._plain(vol) if (bad and mass) else None
+        center = self._center(vol) if (bad and not mass) else None
         for (z0, z1, y0, y1, x0, x1), (rz0, rz1, ry0, ry1, rx0, rx1) in regions:
             if full is not None:
                 src = full[z0:z1, y0:y1, x0:x1]
@@ -510,7 +523,7 @@ class CheckedSurrogate:
                 reg_in = torch.empty((rz1 - rz0, ry1 - ry0, rx1 - rx0), dtype=vol.dtype,
                                      device=vol.device)
                 _array.copy_cast(vol[rz0:rz1, ry0:ry1, rx0:rx1], reg_in)
-                reg = self._plain(reg_in)
+                reg = self._plain(reg_in, center)
                 src = reg[z0 - rz0:z1 - rz0, y0 - ry0:y1 - ry0, x0 - rx0:x1 - rx0]
             _array.copy_cast(src, sur[z0:z1, y0:y1, x0:x1])
         finite = rel[np.isfinite(rel)]

@@ -186,10 +186,14 @@ int hc_separable2d(const hc_array* in, const hc_array* out, const hc_taps* taps0
  * (2-D only).  No work memory is needed.  The passes run per axis, which
  * reorders the reference's passes: exact for every mode except 'constant'
  * with cval != 0, which returns HC_ERR_UNSUPPORTED (run the 1-D passes in
- * the reference order with hc_correlate1d instead). */
+ * the reference order with hc_correlate1d instead).  scratch (optional, >= 64
+ * bytes of device memory): when given and mode is not 'constant', the first
+ * pass subtracts the image's midrange (reduced on the device), which leaves
+ * every Hessian element unchanged in exact arithmetic and shrinks float32
+ * rounding; NULL runs on the image as it is. */
 int hc_hessian_gaussian(const hc_array* in, const hc_array* H, int nH,
                         const hc_taps* taps_s, const hc_taps* taps_d, int mode,
-                        double cval, int order_xy, void* stream);
+                        double cval, int order_xy, void* scratch, void* stream);
 
 /* Symmetric eigenvalues per voxel: feature/corner.py:699-755
  * (_symmetric_compute_eigenvalues).  S: nS = n(n+1)/2 element arrays in
@@ -203,7 +207,11 @@ int hc_sym_eigvals(const hc_array* S, int nS, const hc_array* eig, int sort,
 /* Frangi vesselness: filters/ridges.py:322-476.  taps_s/taps_d hold
  * nsig*ndim kernels (sigma-major).  has_gamma == 0 means gamma=None (gamma
  * from the first sigma, computed on the device; no host synchronisation);
- * otherwise gamma is used as given (any value, 2*gamma^2 as the reference).  work: 6
+ * otherwise gamma is used as given (any value, 2*gamma^2 as the reference).
+ * Except in 'constant' mode the first pass subtracts the image's midrange,
+ * as hc_hessian_gaussian does with a scratch (bytes 32..63), or, when center
+ * (device, one double) is not NULL, that value instead (a sub-volume then
+ * reproduces the values of the whole volume it was cut from).  work: 6
  * (3-D) or 3 (2-D) arrays of the image's shape in the work dtype;
  * scratch: >= 64 bytes of device memory.  out: image shape, work dtype.
  * mode='constant' with cval != 0 returns HC_ERR_UNSUPPORTED (see
@@ -211,7 +219,8 @@ int hc_sym_eigvals(const hc_array* S, int nS, const hc_array* eig, int sort,
 int hc_frangi(const hc_array* in, const hc_array* out, const hc_taps* taps_s,
               const hc_taps* taps_d, int nsig, double alpha, double beta,
               double gamma, int has_gamma, int black_ridges, int mode, double cval,
-              const hc_array* work, int nwork, void* scratch, void* stream);
+              const hc_array* work, int nwork, void* scratch, const double* center,
+              void* stream);
 
 /* Frangi vesselness of one sigma from eigenvalues already sorted ascending
  * by |value| (filters/ridges.py:428-475 after the eigenvalue step): eig holds
@@ -361,6 +370,87 @@ int hc_binary(const hc_array* a, const hc_array* b, const hc_array* out, int op,
 /* Strided copy with dtype cast (cast<> semantics of
  * _ndimage_filters_core.py:227-244): out[...] = cast(in[...]). */
 int hc_copy_cast(const hc_array* in, const hc_array* out, void* stream);
+
+/* ---- entry points that take sigmas (the kernels are built inside) ---- */
+
+/* The 1-D Gaussian correlation kernel of _vendored/_ndimage_filters.py:
+ * 689-717 (_gaussian_kernel1d) in float64: 2 * radius + 1 values into out.
+ * Host-only (no device work).  Evaluated as numpy evaluates it (pairwise
+ * sum, fused multiply-adds in the polynomial), except that exp is the C
+ * library's: values can differ from numpy's in the last bit. */
+int hc_gaussian_kernel1d(double sigma, int order, int64_t radius, double* out);
+
+/* gaussian_filter over every axis: _vendored/_ndimage_filters.py:580-686.
+ * sigma, order, mode: ndim values each; radius: NULL or ndim values (< 0:
+ * int(truncate * sigma + 0.5)); axes with sigma < 1e-15 or radius 0 are
+ * skipped; no axis left copies in to out.  Weights in promote(in, float32).
+ * out: C-contiguous, in's shape, any dtype (intermediates in out's dtype, as
+ * the reference's pass scheduler keeps them).  work: an array like out,
+ * needed when two or more passes run (except the fused 2-D float32 case);
+ * otherwise may be NULL.  Kernels longer than hc_max_inline_taps return
+ * HC_ERR_RESOURCE (run them with hc_correlate1d and a device copy of the
+ * taps). */
+int hc_gaussian_nd(const hc_array* in, const hc_array* out, const double* sigma,
+                   const int* order, const int64_t* radius, const int* mode, double cval,
+                   double truncate, const hc_array* work, void* stream);
+
+/* hessian_matrix(use_gaussian_derivatives=True) from sigmas (ndim values):
+ * feature/corner.py:161-254, the kernels of :218-232 built in the library
+ * (sigma / sqrt(2), truncate 8 if every sigma > 1 else 100, rounded to the
+ * image dtype, trimmed to the common nonzero support), then
+ * hc_hessian_gaussian (same arrays, modes and errors). */
+int hc_hessian_matrix(const hc_array* in, const hc_array* H, int nH, const double* sigma,
+                      int mode, double cval, int order_xy, void* scratch, void* stream);
+
+/* frangi from sigmas: filters/ridges.py:322-476 with the per-sigma kernels
+ * built as hc_hessian_matrix builds them, then hc_frangi (same arguments
+ * otherwise). */
+int hc_frangi_sigmas(const hc_array* in, const hc_array* out, const double* sigmas, int nsig,
+                     double alpha, double beta, double gamma, int has_gamma, int black_ridges,
+                     int mode, double cval, const hc_array* work, int nwork, void* scratch,
+                     void* stream);
+
+/* ---- after the filter: image range, histogram, thresholds, labels ---- */
+
+/* Minimum and maximum of a C-contiguous array (any real dtype; NaN
+ * propagates): the image.min() / image.max() of exposure/exposure.py:51-63
+ * and filters/thresholding.py.  result_dev receives two values of the key
+ * type: double for float inputs, int64 for signed integers, uint64 for
+ * unsigned ones and bool.  work_dev: >= 16 KiB of device memory. */
+int hc_minmax(const hc_array* in, void* result_dev, void* work_dev, int64_t work_bytes,
+              void* stream);
+
+/* Histogram counts (uint64, nbins, zeroed first) of a C-contiguous array:
+ * exposure/exposure.py:66-105, 308-345 (_histogram).  Integer images
+ * (edges_dev NULL): bin = value - lo for values in [lo, lo + nbins), the
+ * bincount of the reference's _bincount_histogram.  Float images: edges_dev
+ * holds nbins + 1 increasing float64 edges; bin i takes edges[i] <= x <
+ * edges[i + 1], the last bin closed, values outside and NaN dropped
+ * (numpy.histogram's rule, which cupy.histogram follows). */
+int hc_histogram(const hc_array* in, int64_t nbins, const double* edges_dev, int64_t lo,
+                 uint64_t* counts_dev, void* stream);
+
+/* One step of threshold_li's float iteration (filters/thresholding.py:
+ * 760-779): over the finite values v = x - vmin (rounded in x's dtype, the
+ * reference's `image -= image_min`), out4_dev = {sum v > t, count v > t,
+ * sum v <= t, count v <= t} in float64.  work_dev: >= 32 KiB. */
+int hc_li_means(const hc_array* in, double vmin, double t, double* out4_dev, void* work_dev,
+                int64_t work_bytes, void* stream);
+
+/* int32 elements of the work buffer hc_label needs for n voxels. */
+int64_t hc_label_work_elems(int64_t n);
+
+/* Connected-component labels: measure/_label.py:21-124 (label in greyscale
+ * mode, _label_kernels.py).  Voxels whose integer key differs from
+ * `background` are foreground; face/edge/corner neighbours within
+ * `connectivity` (generate_binary_structure(ndim, connectivity)) with equal
+ * keys are connected.  Keys: integer values (uint64 as their bit pattern),
+ * floats truncated toward zero (the reference's astype(intp)).  out: int32,
+ * the input's shape, both C-contiguous; labels 1..count numbered in raster
+ * order of each component's first voxel, background 0.  work_dev:
+ * hc_label_work_elems(n) int32; count_dev (optional): the component count. */
+int hc_label(const hc_array* in, int64_t background, int connectivity, const hc_array* out,
+             int32_t* work_dev, int64_t work_elems, int32_t* count_dev, void* stream);
 
 #ifdef __cplusplus
 }

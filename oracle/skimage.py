@@ -16,6 +16,10 @@ Reference files (relative to ``python/cucim/src/cucim/skimage/``):
   filters/edges.py:362-927         scharr, farid, roberts*, laplace, farid_h/v
   restoration/uft.py:429-478       laplacian (spatial kernel)
   filters/_sparse.py:34-118        correlate_sparse
+  exposure/exposure.py:35-345      histogram
+  filters/thresholding.py:298-782  threshold_otsu / yen / li (with scikit-image's
+                                   histogram formulas, restated)
+  measure/_label.py:21-124         label
   color/colorconv.py:160-180,1037-1095  rgb2gray
   feature/corner.py:161-856        hessian_matrix, eigenvalue kernels
   filters/ridges.py:228-476        frangi
@@ -359,6 +363,135 @@ def correlate_sparse(image, kernel, mode="reflect"):
         with np.errstate(invalid="ignore", over="ignore"):
             out = (out + view(t, v)).astype(F)
     return out
+
+
+def histogram(image, nbins=256, source_range="image"):
+    """``exposure/exposure.py:35-345`` (no channel axis, not normalised).
+    Integer images: counts of every value in [min, max] (or the dtype's
+    range) and those values as centers.  Float images: cupy.histogram's
+    edges -- the range widened by 0.5 per side when it is one value, then
+    start + i * ((stop - start) / nbins) in float64, the last edge = stop,
+    rounded to the image's float type -- with numpy's searchsorted rule (the
+    last bin closed); centers (e[:-1] + e[1:]) / 2 in that type."""
+    x = np.asarray(image).reshape(-1)
+    if x.dtype.kind in "iub":
+        if source_range == "image":
+            lo, hi = int(x.min()), int(x.max())
+        elif x.dtype.kind == "b":
+            lo, hi = 0, 1
+        else:
+            lo, hi = int(np.iinfo(x.dtype).min), int(np.iinfo(x.dtype).max)
+        counts = np.bincount((x.astype(np.int64) - lo), minlength=hi - lo + 1)
+        return counts, np.arange(lo, hi + 1)
+    F = x.dtype.type
+    first, last = (float(x.min()), float(x.max())) if source_range == "image" else (-1.0, 1.0)
+    if first == last:
+        first, last = first - 0.5, last + 0.5
+    first, last = float(F(first)), float(F(last))
+    step = (last - first) / nbins
+    e = first + np.arange(nbins + 1, dtype=np.float64) * step
+    e[-1] = last
+    e = e.astype(F)
+    idx = np.searchsorted(e, x, side="right") - 1
+    idx[x == e[-1]] = nbins - 1
+    keep = (idx >= 0) & (idx < nbins) & ~np.isnan(x)
+    counts = np.bincount(idx[keep], minlength=nbins)
+    return counts, ((e[:-1] + e[1:]) / F(2.0)).astype(F)
+
+
+def threshold_otsu(image, nbins=256):
+    """``filters/thresholding.py:355-416`` + scikit-image's Otsu on the
+    float32 counts: argmax over splits of w1 * w2 * (m1 - m2)^2."""
+    x = np.asarray(image)
+    if np.all(x == x.reshape(-1)[0]):
+        return x.reshape(-1)[0]
+    counts, centers = histogram(x, nbins)
+    counts = counts.astype(np.float32)
+    w1 = np.cumsum(counts)
+    w2 = np.cumsum(counts[::-1])[::-1]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        m1 = np.cumsum(counts * centers) / w1
+        m2 = (np.cumsum((counts * centers)[::-1]) / w2[::-1])[::-1]
+    v = w1[:-1] * w2[1:] * (m1[:-1] - m2[1:]) ** 2
+    return centers[np.argmax(v)]
+
+
+def threshold_yen(image, nbins=256):
+    """``filters/thresholding.py:419-474`` + scikit-image's Yen criterion."""
+    counts, centers = histogram(np.asarray(image), nbins)
+    if centers.size == 1:
+        return centers[0]
+    counts = counts.astype(np.float32)
+    pmf = counts / counts.sum()
+    P1 = np.cumsum(pmf)
+    P1_sq = np.cumsum(pmf ** 2)
+    P2_sq = np.cumsum(pmf[::-1] ** 2)[::-1]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        crit = np.log(((P1_sq[:-1] * P2_sq[1:]) ** -1) * (P1[:-1] * (1.0 - P1[:-1])) ** 2)
+    return centers[crit.argmax()]
+
+
+def threshold_li(image, tolerance=None):
+    """``filters/thresholding.py:614-782`` in float64 (means exact)."""
+    x = np.asarray(image).reshape(-1)
+    x = x[~np.isnan(x)] if x.dtype.kind == "f" else x
+    if x.size == 0:
+        return np.nan
+    if np.all(x == x[0]):
+        return x[0]
+    if x.dtype.kind == "f":
+        x = x[np.isfinite(x)]
+        if x.size == 0:
+            return 0.0
+    image_min = x.min()
+    v = (x - image_min).astype(x.dtype)
+    if v.dtype.kind in "iu":
+        tolerance = tolerance or 0.5
+    else:
+        tolerance = tolerance or float(np.min(np.diff(np.unique(v))) / 2)
+    t_next = float(np.mean(v.astype(np.float64)))
+    t_curr = -2 * tolerance
+    eps = 100 * np.finfo(float).eps
+    while abs(t_next - t_curr) > tolerance:
+        t_curr = t_next
+        fg = v > t_curr
+        mean_fore = float(np.mean(v[fg].astype(np.float64)))
+        mean_back = float(np.mean(v[~fg].astype(np.float64)))
+        if mean_back == 0:
+            break
+        t_next = (mean_back - mean_fore) / (math.log(mean_back + eps) - math.log(mean_fore + eps))
+    return t_next + float(image_min)
+
+
+def label(label_image, background=0, connectivity=None):
+    """``measure/_label.py:21-124``: connected regions of equal value
+    (scipy.ndimage.label on each value's mask with
+    generate_binary_structure(ndim, connectivity)), background 0, regions
+    numbered in raster order of their first element."""
+    import scipy.ndimage as sndi
+
+    x = np.asarray(label_image)
+    if x.dtype.kind not in "bui":
+        x = x.astype(np.intp)
+    nd = x.ndim
+    connectivity = nd if connectivity is None else connectivity
+    st = sndi.generate_binary_structure(nd, connectivity)
+    comp = np.zeros(x.shape, np.int64)
+    nxt = 0
+    for v in np.unique(x):
+        if v == background:
+            continue
+        lab, k = sndi.label(x == v, structure=st)
+        comp[lab > 0] = lab[lab > 0] + nxt
+        nxt += k
+    flat = comp.reshape(-1)
+    order = np.zeros(nxt + 1, np.int64)
+    seen = flat[flat > 0]
+    _, first = np.unique(seen, return_index=True)
+    ids = np.unique(seen)
+    rank = np.argsort(np.argsort(first))
+    order[ids] = rank + 1
+    return order[flat].reshape(x.shape).astype(np.int32), nxt
 
 
 def rgb2gray(rgb, *, channel_axis=-1):

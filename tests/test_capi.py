@@ -67,7 +67,7 @@ def test_argument_errors_return_before_launch():
                               _lib.ALGO_TILED, None) == _lib.ERR_RESOURCE
     # the fused Hessian only takes 2-D / 3-D float images
     img = _arr((4, 5, 6, 7))
-    assert lib.hc_hessian_gaussian(ctypes.byref(img), None, 0, None, None, 0, 0.0, 0,
+    assert lib.hc_hessian_gaussian(ctypes.byref(img), None, 0, None, None, 0, 0.0, 0, None,
                                    None) == _lib.ERR_VALUE
     with pytest.raises(ValueError):
         _lib.check(_lib.ERR_VALUE)
@@ -114,3 +114,28 @@ def test_batch2d_plan_counts():
     assert lib.hc_batch2d_plan(tiles, len(shapes), 40, 16, None, 0) == -1
     small = (ctypes.c_int32 * 3)()
     assert lib.hc_batch2d_plan(tiles, len(shapes), 13, 16, small, 3) == -1
+
+
+def test_library_gaussian_kernel_matches_python():
+    """hc_gaussian_kernel1d (host code) restates _gaussian_kernel1d
+    (_vendored/_ndimage_filters.py:689-717): float64 within a few ulp of
+    numpy's evaluation (the exp differs), float32-rounded taps equal, and
+    the Hessian kernels of every sigma of the default Frangi range equal
+    after rounding."""
+    from cucim_amd.skimage._vendored._ndimage_filters import _gaussian_kernel1d
+
+    lib = _lib.load_library()
+    worst = 0
+    for sigma in list(np.arange(0.3, 12.0, 0.173)) + [1 / np.sqrt(2) * s for s in range(1, 10)]:
+        for order in (0, 1, 2, 3):
+            r = int(8 * sigma + 0.5)
+            out = np.empty(2 * r + 1)
+            assert lib.hc_gaussian_kernel1d(float(sigma), order, r, out.ctypes.data) == 0
+            ref = _gaussian_kernel1d(float(sigma), order, r)
+            scale = np.abs(ref).max()
+            worst = max(worst, float(np.abs(out - ref).max() / scale))
+            if order <= 2:
+                np.testing.assert_array_equal(out.astype(np.float32), ref.astype(np.float32))
+    assert worst < 1e-14, worst
+    out = np.empty(3)
+    assert lib.hc_gaussian_kernel1d(1.0, -1, 1, out.ctypes.data) == _lib.ERR_VALUE

@@ -145,3 +145,73 @@ def test_difference_of_gaussians_errors(gpu):
         difference_of_gaussians(_t(np.ones((3, 3))), 3, 2)
     with pytest.raises(ValueError):
         difference_of_gaussians(_t(np.ones((3, 3))), (1, 5), (2, 4))
+
+
+def _sym_elems(mats):
+    """Element images of the stacked matrices, shaped (count, 1, ..., 1) so
+    that the image rank equals the matrix order, as the reference's
+    _symmetric_image assumes."""
+    n = mats.shape[-1]
+    from itertools import combinations_with_replacement as cwr
+
+    shp = (mats.shape[0],) + (1,) * (n - 1)
+    return [np.ascontiguousarray(mats[..., r, c]).reshape(shp) for r, c in cwr(range(n), 2)]
+
+
+@pytest.mark.parametrize("n", [1, 4, 5, 8])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("sort,abs_sort", [("descending", False), ("ascending", False),
+                                           ("ascending", True), ("descending", True)])
+def test_eigvals_nd_vs_eigvalsh(gpu, n, dtype, sort, abs_sort):
+    """The eigvalsh branch of _symmetric_compute_eigenvalues
+    (feature/corner.py:744-755) for matrix orders other than 2 and 3:
+    random symmetric matrices, plus exactly diagonal ones (ties included) and
+    rank-1 ones, against numpy.linalg.eigvalsh in float64."""
+    from cucim_amd.skimage.feature.corner import _symmetric_compute_eigenvalues
+
+    rng = np.random.default_rng(40 + n)
+    m = rng.standard_normal((300, n, n))
+    m = m + np.swapaxes(m, -1, -2)
+    d = np.zeros((40, n, n))
+    idx = np.arange(n)
+    d[:, idx, idx] = rng.integers(-3, 4, (40, n))  # ties, zeros, +-pairs
+    v = rng.standard_normal((40, n))
+    r1 = v[:, :, None] * v[:, None, :]
+    mats = np.concatenate([m, d, r1]).astype(dtype)
+    elems = _sym_elems(mats)
+    got = _np(_symmetric_compute_eigenvalues([_t(e) for e in elems], sort=sort,
+                                             abs_sort=abs_sort))
+    exp = osk.symmetric_compute_eigenvalues([e.astype(np.float64) for e in elems],
+                                            sort=sort, abs_sort=abs_sort)
+    got = got.reshape(n, -1)
+    exp = exp.reshape(n, -1)
+    assert got.dtype == dtype and got.shape == (n, mats.shape[0])
+    scale = np.abs(exp).max(axis=0, keepdims=True)
+    tol = 2e-6 if dtype == np.float32 else 1e-12
+    assert np.all(np.abs(got - exp) <= tol * scale)
+    # the diagonal block is exact, ties in the stable order
+    np.testing.assert_array_equal(got[:, 300:340], exp[:, 300:340].astype(dtype))
+
+
+def test_hessian_eigvals_4d(gpu):
+    """hessian_matrix_eigvals and structure_tensor_eigenvalues of a 4-D image
+    (the reference raises nothing here; it stacks the 10 elements and calls
+    eigvalsh)."""
+    from cucim_amd.skimage.feature import (hessian_matrix, hessian_matrix_eigvals,
+                                           structure_tensor, structure_tensor_eigenvalues)
+
+    rng = np.random.default_rng(44)
+    import scipy.ndimage as sndi
+
+    x = sndi.gaussian_filter(rng.standard_normal((9, 10, 11, 12)), 1.0).astype(np.float32)
+    H = hessian_matrix(_t(x), sigma=1.5, use_gaussian_derivatives=True)
+    assert len(H) == 10
+    got = _np(hessian_matrix_eigvals(H))
+    exp = osk.hessian_matrix_eigvals([_np(h).astype(np.float64) for h in H])
+    assert got.shape == (4,) + x.shape
+    assert np.abs(got - exp).max() <= 2e-6 * np.abs(exp).max()
+    A = structure_tensor(_t(x), sigma=1.0)
+    got = _np(structure_tensor_eigenvalues(A))
+    exp = osk.structure_tensor_eigenvalues([_np(a).astype(np.float64) for a in A])
+    assert np.abs(got - exp).max() <= 2e-6 * np.abs(exp).max()
+    assert np.all(np.diff(got, axis=0) <= 0)
