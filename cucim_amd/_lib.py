@@ -63,6 +63,7 @@ ENTRY_POINTS = (
     "hc_hessian_matrix",
     "hc_frangi_sigmas",
     "hc_minmax",
+    "hc_minmax_noalloc",
     "hc_histogram",
     "hc_li_means",
     "hc_label_work_elems",
@@ -151,6 +152,7 @@ _SIGNATURES = {
     "hc_hessian_matrix": ([_A, _A, _i, _v, _i, _d, _i, _v, _v], _i),
     "hc_frangi_sigmas": ([_A, _A, _v, _i, _d, _d, _d, _i, _i, _i, _d, _A, _i, _v, _v], _i),
     "hc_minmax": ([_A, _v, _v, ctypes.c_int64, _v], _i),
+    "hc_minmax_noalloc": ([_A, _v, _v], _i),
     "hc_histogram": ([_A, ctypes.c_int64, _v, ctypes.c_int64, _v, _v], _i),
     "hc_li_means": ([_A, _d, _d, _v, _v, ctypes.c_int64, _v], _i),
     "hc_label_work_elems": ([ctypes.c_int64], ctypes.c_int64),

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "hc_common.h"
@@ -413,6 +414,33 @@ __global__ __launch_bounds__(256) void k_label_final(int32_t* __restrict__ y, in
   }
 }
 
+// Scratch of the allocation-free hc_minmax_noalloc: one (min, max) pair of
+// 8-byte keys per partial block.  Each GPU holds its own copy until the
+// process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device.
+constexpr int kNoAllocScratchBytes = kRedMaxBlocks * 16;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up on the first call
+// per device (with that device current) and kept.
+int scratch_ptr(void** out) {
+  int dev = 0;
+  HC_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(HC_ERR_UNSUPPORTED, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return HC_OK;
+}
+
 int red_blocks(int64_t n) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(kRedMaxBlocks, (n + 2047) / 2048));
 }
@@ -458,6 +486,16 @@ int hc_minmax(const hc_array* in, void* result_dev, void* work_dev, int64_t work
 #undef HC_MINMAX
   HC_HIP(hipGetLastError());
   return HC_OK;
+}
+
+int hc_minmax_noalloc(const hc_array* in, void* result_dev, void* stream) {
+  if (!in || !result_dev) return fail(HC_ERR_VALUE, "null argument");
+  if (numel(in) > 0 && red_blocks(numel(in)) * 16 > kNoAllocScratchBytes)
+    return fail(HC_ERR_VALUE, "partials exceed the static scratch");
+  void* p = nullptr;
+  const int rc = scratch_ptr(&p);
+  if (rc != HC_OK) return rc;
+  return hc_minmax(in, result_dev, p, kNoAllocScratchBytes, stream);
 }
 
 int hc_histogram(const hc_array* in, int64_t nbins, const double* edges_dev, int64_t lo,

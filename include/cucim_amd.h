@@ -420,6 +420,14 @@ int hc_frangi_sigmas(const hc_array* in, const hc_array* out, const double* sigm
 int hc_minmax(const hc_array* in, void* result_dev, void* work_dev, int64_t work_bytes,
               void* stream);
 
+/* hc_minmax without a work argument: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops).  The partial results go to a static 16 KiB device array of
+ * the library (one per GPU, held until the process ends; its address is
+ * looked up once per device on the first call).  Allocates nothing.  Calls
+ * that overlap on one GPU race on that array. */
+int hc_minmax_noalloc(const hc_array* in, void* result_dev, void* stream);
+
 /* Histogram counts (uint64, nbins, zeroed first) of a C-contiguous array:
  * exposure/exposure.py:66-105, 308-345 (_histogram).  Integer images
  * (edges_dev NULL): bin = value - lo for values in [lo, lo + nbins), the
