@@ -1,6 +1,9 @@
 """Compile the HIP sources into ``cucim_amd/lib/libcucim_amd.so`` (gfx950).
 
-Run ``python -m cucim_amd._build`` or call :func:`build`.  Objects go to
+Run ``python -m cucim_amd._build`` or call :func:`build`.
+``python -m cucim_amd._build --sanitize-host DRIVER.cpp OUTDIR`` builds the
+host code with AddressSanitizer and UndefinedBehaviorSanitizer instead
+(:func:`build_host_sanitized`; tests/host_selftest.cpp is such a driver).  Objects go to
 ``_build/`` at the repository root; the shared library is built in-tree so
 that it travels with the source tree.
 """
@@ -70,6 +73,53 @@ def build(verbose=False):
     return LIB
 
 
+SANITIZE = ["-fsanitize=address", "-fsanitize=undefined", "-fno-sanitize-recover=undefined"]
+
+
+def build_host_sanitized(driver, outdir):
+    """Host-only build of every source with AddressSanitizer and
+    UndefinedBehaviorSanitizer, linked with ``driver`` (a C++ file with a
+    main()) into ``outdir/<driver stem>``.  No device code is compiled: the
+    executable runs the host logic of the C ABI (tap construction, planning,
+    argument checks) on a machine without a GPU; a kernel launch from it
+    fails with a HIP error.  Run it with ASAN_OPTIONS=detect_leaks=0 (the HIP
+    runtime keeps allocations until exit)."""
+    os.makedirs(outdir, exist_ok=True)
+    host = [x for f in SANITIZE for x in ("-Xarch_host", f)]
+
+    def one(src):
+        obj = os.path.join(outdir, os.path.splitext(src)[0] + ".o")
+        cmd = [_hipcc(), "-fgpu-rdc", "--offload-host-only", "-O1", "-g", "-std=c++17",
+               "-fPIC", f"-I{INC}", *host, "-c", os.path.join(SRC, src), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc (host, sanitized) failed for {src}:\n{r.stderr}")
+        return obj
+
+    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+        objs = list(ex.map(one, SOURCES))
+    clang = os.path.join(os.path.dirname(os.path.realpath(_hipcc())), "..", "lib", "llvm", "bin",
+                         "clang++")
+    if not os.path.exists(clang):
+        clang = "/opt/rocm/lib/llvm/bin/clang++"
+    stem = os.path.splitext(os.path.basename(driver))[0]
+    dobj = os.path.join(outdir, stem + ".o")
+    r = subprocess.run([clang, "-O1", "-g", "-std=c++17", f"-I{INC}", *SANITIZE, "-c", driver,
+                        "-o", dobj], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"clang++ failed for {driver}:\n{r.stderr}")
+    exe = os.path.join(outdir, stem)
+    r = subprocess.run([_hipcc(), "-fgpu-rdc", "--hip-link", f"--offload-arch={ARCH}",
+                        "-fsanitize=address,undefined", *objs, dobj, "-o", exe],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link (sanitized) failed:\n{r.stderr}")
+    return exe
+
+
 if __name__ == "__main__":
+    if len(sys.argv) == 4 and sys.argv[1] == "--sanitize-host":
+        print(build_host_sanitized(sys.argv[2], sys.argv[3]))
+        sys.exit(0)
     build(verbose=True)
     sys.exit(0)

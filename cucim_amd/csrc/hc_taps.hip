@@ -173,6 +173,7 @@ int hc_gaussian_nd(const hc_array* in, const hc_array* out, const double* sigma,
                    double truncate, const hc_array* work, void* stream) {
   if (!in || !out || !sigma || !order || !mode) return fail(HC_ERR_VALUE, "null argument");
   const int nd = in->ndim;
+  if (nd < 1 || nd > HC_MAX_NDIM) return fail(HC_ERR_VALUE, "rank out of range");
   const bool f64 = weights_f64(in->dtype);
   std::vector<AxisKernel> ks;
   std::vector<int> axes;

@@ -1488,19 +1488,47 @@ __global__ __launch_bounds__(256) void k_center_reduce(const void* __restrict__ 
   __syncthreads();
   double lo = INFINITY, hi = -INFINITY;
   bool bad = false;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * 256) {
-    double v;
-    if constexpr (std::is_same<T, void>::value)
-      v = load_as<double>(x, i, dt);
-    else
-      v = (double)reinterpret_cast<const T*>(x)[i];
+  auto take = [&](double v) {
     if (!isfinite(v)) {
       bad = true;
     } else {
       lo = v < lo ? v : lo;
       hi = v > hi ? v : hi;
     }
+  };
+  int64_t i0 = 0;
+  if constexpr (std::is_same<T, float>::value) {
+    // 16-byte loads, four per lane in flight, over the 16-byte aligned part
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    const float* xf = reinterpret_cast<const float*>(x);
+    const int64_t head = std::min<int64_t>(n, ((16 - ((uintptr_t)xf & 15)) & 15) / 4);
+    const int64_t nv = (n - head) / 4;
+    const F4* xv = reinterpret_cast<const F4*>(xf + head);
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < nv; c += 4 * stride) {
+      F4 t[4];
+#pragma unroll
+      // a repeated vector (past the end) changes neither minimum nor maximum
+      for (int u = 0; u < 4; ++u) t[u] = c + u * stride < nv ? xv[c + u * stride] : xv[c];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) take((double)t[u][j]);
+    }
+    // the unaligned head and the tail (fewer than 4 values each)
+    const int64_t t0 = head + 4 * nv;
+    if (blockIdx.x == 0 && threadIdx.x < head) take((double)xf[threadIdx.x]);
+    if (blockIdx.x == 0 && threadIdx.x < n - t0) take((double)xf[t0 + threadIdx.x]);
+    i0 = n;
+  }
+  for (int64_t i = i0 + (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    double v;
+    if constexpr (std::is_same<T, void>::value)
+      v = load_as<double>(x, i, dt);
+    else
+      v = (double)reinterpret_cast<const T*>(x)[i];
+    take(v);
   }
   if (bad) sbad = 1;
   const int tid = threadIdx.x;

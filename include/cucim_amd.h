@@ -100,8 +100,9 @@ int hc_timing_collect(char* names, int names_len, double* ms, int max_records);
 
 /* Tile-shape knobs (no counterpart in the reference, whose shared-memory
  * kernel fixes its tiles in filters/_separable_filtering.py:17-91).  A value
- * of 0 restores the built-in heuristic.  Set by the autotuner in
- * cucim_amd/autotune.py; process-wide, read at each launch. */
+ * of 0 restores the built-in heuristic.  Process-wide, read at each
+ * launch; the batched 2-D driver's autotuner sets HC_TUNE_SEP2D_TY-like
+ * choices per plan (cucim_amd/batch.py). */
 typedef enum hc_tune_knob {
   HC_TUNE_CONTIG_SEG = 0,   /* contiguous-axis correlate: outputs per wave
                                segment, 256..4096, multiple of 256 */
