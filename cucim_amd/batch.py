@@ -6,7 +6,8 @@ per tile.  Here a plan binds a list of input and output images, builds a
 work-item table once (``hc_batch2d_plan``), and runs the whole batch as one
 persistent launch that can be captured into a HIP graph and replayed.  Tile
 shapes (output rows per work item) are chosen by measuring the candidates
-(``autotune``); choices are remembered per workload signature.
+(``autotune``); choices are remembered per workload signature, in memory and
+in a JSON file that later processes read.
 
 GrayGaussSobelBatch: per image, ``filters.sobel(filters.gaussian(
 color.rgb2gray(rgb), sigma, mode=gauss_mode, cval=gauss_cval,
@@ -16,12 +17,15 @@ filters/edges.py:148-241), each output equal to the three separate calls.
 """
 
 import ctypes
+import hashlib
+import json
 import math
+import os
 
 import numpy as np
 import torch
 
-from . import _array, _lib
+from . import _array, _lib, _trace
 from .skimage._vendored import _ndimage_filters as _F
 from .skimage._vendored._ndimage_util import _mode_code
 
@@ -36,8 +40,73 @@ class HCTile2d(ctypes.Structure):
     ]
 
 
-# workload signature -> chosen item rows
+# workload signature -> chosen item rows; persisted across processes in a
+# JSON file (CUCIM_AMD_TUNE_CACHE, default ~/.cache/cucim_amd/autotune.json;
+# "off" disables the file)
 _TUNED = {}
+_TUNED_LOADED = False
+
+
+def _tune_path():
+    p = os.environ.get("CUCIM_AMD_TUNE_CACHE")
+    if p is not None:
+        return None if p.strip().lower() in ("", "0", "off", "none") else p
+    return os.path.join(os.path.expanduser("~"), ".cache", "cucim_amd", "autotune.json")
+
+
+def _tune_read(path):
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return {}
+    ent = data.get("entries") if isinstance(data, dict) else None
+    if not isinstance(ent, dict):
+        return {}
+    return {k: v for k, v in ent.items() if isinstance(k, str) and isinstance(v, int)}
+
+
+def tune_load():
+    """Merge the persisted table into the in-memory one (entries chosen in
+    this process win)."""
+    global _TUNED_LOADED
+    _TUNED_LOADED = True
+    path = _tune_path()
+    if path is None:
+        return
+    for k, v in _tune_read(path).items():
+        _TUNED.setdefault(k, v)
+
+
+def tune_save():
+    """Write the table, merged with what other processes saved meanwhile
+    (atomic replace)."""
+    path = _tune_path()
+    if path is None:
+        return
+    merged = _tune_read(path)
+    merged.update(_TUNED)
+    try:
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        tmp = f"{path}.{os.getpid()}.tmp"
+        with open(tmp, "w") as f:
+            json.dump({"version": 1, "entries": merged}, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+
+
+def _tuned(sig, default):
+    if not _TUNED_LOADED:
+        tune_load()
+    return _TUNED.get(sig, default)
+
+
+def _arch():
+    try:
+        return torch.cuda.get_device_properties(torch.cuda.current_device()).gcnArchName
+    except (RuntimeError, AssertionError, AttributeError):
+        return "unknown"
 
 
 def _time_ms(fn, reps):
@@ -102,13 +171,17 @@ class GrayGaussSobelBatch:
         self.smooth3 = np.ascontiguousarray(smooth[::-1], dtype=np.float64)
         self.pixels = sum(int(x.shape[0]) * int(x.shape[1]) for x in self.inputs)
         self._tables = {}
-        self.rows = rows if rows is not None else _TUNED.get(self.signature(), 16)
+        self.rows = rows if rows is not None else _tuned(self.signature(), 16)
         if self.rows not in self.ROWS:
             raise ValueError("rows must be 16 or 32")
 
     def signature(self):
-        shapes = tuple(sorted((int(x.shape[0]), int(x.shape[1])) for x in self.inputs))
-        return ("gray_gauss_sobel", self.taps.w.size, hash(shapes))
+        """Workload key of the tuning table: pipeline, kernel length, modes,
+        device architecture and a digest of the sorted image shapes."""
+        shapes = sorted((int(x.shape[0]), int(x.shape[1])) for x in self.inputs)
+        digest = hashlib.sha1(repr(shapes).encode()).hexdigest()[:16]
+        return (f"gray_gauss_sobel|K={self.taps.w.size}|modes={self.gauss_mode},"
+                f"{self.edge_mode}|{_arch()}|n={len(shapes)}|{digest}")
 
     def _table(self, rows):
         if rows in self._tables:
@@ -136,6 +209,7 @@ class GrayGaussSobelBatch:
         self._tables[rows] = tab
         return tab
 
+    @_trace.annotate("batch2d.run")
     def run(self):
         """Launch the whole batch on the current stream; returns the outputs."""
         tiles_dev, items_dev, total = self._table(self.rows)
@@ -160,6 +234,7 @@ class GrayGaussSobelBatch:
                 best = (rows, ms)
         self.rows = best[0]
         _TUNED[self.signature()] = self.rows
+        tune_save()
         return {"rows": self.rows, "ms": best[1]}
 
     def capture(self):
@@ -210,6 +285,7 @@ class FrangiBatch3D:
                                self.gamma, self.black_ridges)
         _lib.check(rc, "frangi")
 
+    @_trace.annotate("batch3d.run")
     def run(self):
         for i in range(len(self.volumes)):
             self.run_one(i)

@@ -6,6 +6,7 @@ from collections.abc import Iterable
 import numpy as np
 
 from ... import _array
+from .._vendored import _ndimage_filters as ndi_filters
 from .._vendored import ndimage as ndi
 from .utils import _supported_float_type, astype, convert_to_float
 
@@ -15,6 +16,12 @@ def gaussian(image, sigma=1, mode="nearest", cval=0, preserve_range=False,
     """Multi-dimensional Gaussian filter (skimage semantics: default
     ``mode='nearest'``, integer images converted with img_as_float unless
     ``preserve_range``)."""
+    if out is None and channel_axis is None:
+        # a float32 image through a recorded launch plan (same result as the
+        # path below: no conversion, a new float32 output, the fused filter)
+        r = ndi_filters._gaussian_fast(image, sigma, 0, mode, cval, truncate, None, (0, 1))
+        if r is not None:
+            return r
     sigma_msg = "Sigma values less than zero are not valid"
     if not isinstance(sigma, Iterable):
         if sigma < 0:

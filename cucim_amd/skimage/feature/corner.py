@@ -8,7 +8,7 @@ from warnings import warn
 import numpy as np
 import torch
 
-from ... import _array, _lib
+from ... import _array, _lib, _trace
 from .._shared.utils import _supported_float_type, astype
 from .._vendored import ndimage as ndi
 from .._vendored._ndimage_filters import _gaussian_kernel1d, _trimmed
@@ -115,6 +115,7 @@ def _gradient(f, axis):
     return out
 
 
+@_trace.annotate("hessian_matrix")
 def hessian_matrix(image, sigma=1, mode="constant", cval=0, order="rc",
                    use_gaussian_derivatives=None):
     """Hessian matrix elements (``corner.py:257-384``)."""
@@ -180,6 +181,7 @@ def _symmetric_compute_eigenvalues(S_elems, sort="descending", abs_sort=False):
     return eigs
 
 
+@_trace.annotate("hessian_matrix_eigvals")
 def hessian_matrix_eigvals(H_elems):
     """Eigenvalues of the Hessian, decreasing (``corner.py:824-856``)."""
     return _symmetric_compute_eigenvalues(H_elems)
@@ -202,6 +204,7 @@ def _compute_derivatives(image, mode="constant", cval=0):
     return [ndi.sobel(image, axis=i, mode=mode, cval=cval) for i in range(image.dim())]
 
 
+@_trace.annotate("structure_tensor")
 def structure_tensor(image, sigma=1, mode="constant", cval=0, order="rc"):
     """Structure tensor (``corner.py:55-158``): the Gaussian-smoothed
     products of the Sobel derivatives, in ``combinations_with_replacement``
@@ -232,6 +235,7 @@ def structure_tensor(image, sigma=1, mode="constant", cval=0, order="rc"):
     return out
 
 
+@_trace.annotate("structure_tensor_eigenvalues")
 def structure_tensor_eigenvalues(A_elems):
     """Eigenvalues of the structure tensor, decreasing (``corner.py:786-821``)."""
     return _symmetric_compute_eigenvalues(A_elems)

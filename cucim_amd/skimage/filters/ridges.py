@@ -7,7 +7,7 @@ from warnings import warn
 import numpy as np
 import torch
 
-from ... import _array, _lib
+from ... import _array, _lib, _trace
 from .._shared.utils import _supported_float_type, astype, check_nD
 from .._vendored._ndimage_util import _mode_code
 from ..feature.corner import (
@@ -80,6 +80,7 @@ def _frangi_sequential(image, sigmas, alpha, beta, gamma, black_ridges, mode, cv
     return out
 
 
+@_trace.annotate("frangi")
 def frangi(image, sigmas=range(1, 10, 2), scale_range=None, scale_step=None, alpha=0.5,
            beta=0.5, gamma=None, black_ridges=True, mode="reflect", cval=0):
     """Frangi vesselness filter (2-D / 3-D), maximum over ``sigmas``."""
@@ -121,6 +122,7 @@ def _ridge_input(image, black_ridges):
     return image, float_dtype
 
 
+@_trace.annotate("sato")
 def sato(image, sigmas=range(1, 10, 2), black_ridges=True, mode="reflect", cval=0):
     """Sato tubeness filter (``ridges.py:148-225``): per sigma, the geometric
     mean of the positive Hessian eigenvalues other than the smallest, times
@@ -139,6 +141,7 @@ def sato(image, sigmas=range(1, 10, 2), black_ridges=True, mode="reflect", cval=
     return out
 
 
+@_trace.annotate("meijering")
 def meijering(image, sigmas=range(1, 10, 2), alpha=None, black_ridges=True, mode="reflect",
               cval=0):
     """Meijering neuriteness filter (``ridges.py:50-145``): per sigma, the
@@ -167,6 +170,7 @@ def meijering(image, sigmas=range(1, 10, 2), alpha=None, black_ridges=True, mode
     return out
 
 
+@_trace.annotate("hessian")
 def hessian(image, sigmas=range(1, 10, 2), scale_range=None, scale_step=None, alpha=0.5,
             beta=0.5, gamma=15, black_ridges=True, mode="reflect", cval=0):
     """Hybrid Hessian filter (``ridges.py:479-565``): Frangi with gamma = 15

@@ -15,7 +15,7 @@ from warnings import warn
 import numpy as np
 import torch
 
-from ... import _array, _lib
+from ... import _array, _lib, _trace
 from ..exposure.exposure import _range, histogram
 
 __all__ = ["threshold_otsu", "threshold_yen", "threshold_li"]
@@ -83,6 +83,7 @@ def _otsu_from_hist(counts, centers):
     return centers[np.argmax(variance12)]
 
 
+@_trace.annotate("threshold_otsu")
 def threshold_otsu(image=None, nbins=256, *, hist=None):
     """Otsu's threshold (``thresholding.py:355-416``)."""
     if image is not None:
@@ -99,6 +100,7 @@ def threshold_otsu(image=None, nbins=256, *, hist=None):
         return _scalar(_otsu_from_hist(counts, centers))
 
 
+@_trace.annotate("threshold_yen")
 def threshold_yen(image=None, nbins=256, *, hist=None):
     """Yen's threshold (``thresholding.py:419-474``)."""
     counts, centers = _validate_image_histogram(image, hist, nbins)
@@ -113,6 +115,7 @@ def threshold_yen(image=None, nbins=256, *, hist=None):
     return _scalar(centers[crit.argmax()])
 
 
+@_trace.annotate("threshold_li")
 def threshold_li(image, *, tolerance=None, initial_guess=None, iter_callback=None):
     """Li's iterative minimum cross entropy threshold
     (``thresholding.py:614-782``).  NaN values are ignored; a single-valued

@@ -6,7 +6,7 @@ hooking, then the components ranked in raster order by a block scan."""
 import numpy as np
 import torch
 
-from ... import _array, _lib
+from ... import _array, _lib, _trace
 
 __all__ = ["label"]
 
@@ -30,6 +30,7 @@ def _background_key(background, dt):
     return v
 
 
+@_trace.annotate("label")
 def label(label_image, background=None, return_num=False, connectivity=None):
     """Label connected regions of equal value (``_label.py:21-124``):
     neighbours within ``connectivity`` (1 .. ndim, default ndim) that hold

@@ -29,7 +29,7 @@ import os
 import numpy as np
 import torch
 
-from . import _array, _lib
+from . import _array, _lib, _trace
 
 # (in channels, out channels, activation): 1 = GELU, 0 = none (training),
 # 2 = clamp to [0, 1] (inference of the last layer)
@@ -561,6 +561,7 @@ class SurrogateBatch3D:
                "pad")
         self.model.forward(x, bufs=self.bufs, out=self.outputs[i])
 
+    @_trace.annotate("surrogate.run")
     def run(self):
         for i in range(len(self.volumes)):
             self.run_one(i)

@@ -74,6 +74,33 @@ def test_gaussian_out_and_errors(gpu):
         gaussian(x, 1.0, out=torch.empty(x.shape, dtype=torch.int32, device="cuda"))
 
 
+@pytest.mark.parametrize("mode", ["reflect", "nearest", "constant"])
+def test_gaussian_launch_plan(gpu, mode):
+    """Repeated float32 2-D calls go through a recorded launch plan; each
+    result equals the planless path's (out= given) bit for bit, for new
+    inputs of the same geometry too, through skimage and ndimage."""
+    from cucim_amd.skimage._vendored import _ndimage_filters as F
+    from cucim_amd.skimage.filters import gaussian
+
+    rng = np.random.default_rng(5)
+    F._FAST_GAUSS.clear()
+    xs = [_t(rng.random((67, 130)).astype(np.float32)) for _ in range(3)]
+    for x in xs:
+        ref = torch.empty_like(x)
+        gaussian(x, 1.7, mode=mode, cval=0.5, out=ref)
+        got = gaussian(x, 1.7, mode=mode, cval=0.5)
+        np.testing.assert_array_equal(_np(got), _np(ref))
+        g2 = F.gaussian_filter(x, 1.7, mode=mode, cval=0.5)
+        ref2 = torch.empty_like(x)
+        F.gaussian_filter(x, 1.7, mode=mode, cval=0.5, output=ref2)
+        np.testing.assert_array_equal(_np(g2), _np(ref2))
+    assert len(F._FAST_GAUSS) == 2  # one per entry (skimage axes (0, 1), ndimage None)
+    # another geometry or sigma records its own plan; a view is not planned
+    gaussian(xs[0][:, :100], 1.7, mode=mode, cval=0.5)
+    gaussian(xs[0], 2.5, mode=mode, cval=0.5)
+    assert len(F._FAST_GAUSS) == 3
+
+
 def test_sobel_kats(gpu):
     from cucim_amd.skimage.filters import sobel
 
