@@ -37,12 +37,13 @@ def gaussian(image, sigma=1, mode="nearest", cval=0, preserve_range=False,
     float_dtype = _supported_float_type(image.dtype)
     image = astype(image, float_dtype)
     if out is None:
-        out = _array.empty(image.shape, float_dtype)
-    else:
-        out_t = _array.as_tensor(out)
-        if _array.np_dtype(out_t.dtype).kind != "f":
-            raise ValueError(f"dtype of `out` must be floating point; got {out_t.dtype!r}.")
+        # a new output of the float type (through a recorded launch plan on
+        # repeated float32 2-D calls)
+        return ndi.gaussian_filter(image, sigma, mode=mode, cval=cval, truncate=truncate,
+                                   axes=axes)
     out_t = _array.as_tensor(out)
+    if _array.np_dtype(out_t.dtype).kind != "f":
+        raise ValueError(f"dtype of `out` must be floating point; got {out_t.dtype!r}.")
     ndi.gaussian_filter(image, sigma, output=out_t, mode=mode, cval=cval,
                         truncate=truncate, axes=axes)
     return out
