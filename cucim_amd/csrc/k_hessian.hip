@@ -256,7 +256,17 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 // Row stride (floats) of the matrix-core passes' line buffers: 16, so that a
 // wave's B fragment (4 rows x 16 lines) is 64 consecutive words (no bank
 // conflict) and two buffers of a 256-voxel line fit four work-groups per CU.
+// The chain pass stages strided lines in this position-major layout (16
+// adjacent lines are 16 consecutive words in memory too).
 constexpr int kMfS = 16;
+
+// The final pass stages contiguous lines line-major instead: line l's row r
+// at X[l * LS + r], LS = 2 (mod 4) and >= E.  Then a lane's 4 consecutive
+// positions of one line are two 8-byte words, and the staging stores (16
+// lines at one position per 16 lanes), the B fragments (16 lines x 2 rows
+// per 32 lanes) and the halo copies fall in distinct banks of the 32 that
+// serve each half-wave.
+__host__ __device__ constexpr int lm_stride(int E) { return E + ((6 - E % 4) % 4); }
 
 // Length of the zero-padded tap arrays: the passes read wp[15 + 4 s + k - i]
 // for steps s < ceil((Km + 15) / 4), k < 4, i < 16.
@@ -278,11 +288,12 @@ constexpr int kMfmaNpf = 16;
 
 // Blocks b = wave + NW i (i < NBW) of 16 positions: acc[s][i] for the NS
 // weight sets wp[s] (zero-padded: trimmed tap t at wp[15 + t]).  The A
-// fragment of step (d, u) serves every block of the wave.
+// fragment of a step of 4 inputs serves every block of the wave.
 template <int NBW, int NS, int NW>
 __device__ __forceinline__ void chain_pass_mfma(f32x4_t (&acc)[NS][NBW], const float* X,
                                                 const float* const (&wp)[NS], int nd,
-                                                int nblk, int wv, int lane, int Km) {
+                                                int nblk, int wv, int lane, int Km, int ps,
+                                                int ls) {
   // steps of 4 inputs that the band of 16 outputs needs (the last 16-row
   // block is cut short)
   const int nsteps = (Km + 15 + 3) / 4;
@@ -296,23 +307,37 @@ __device__ __forceinline__ void chain_pass_mfma(f32x4_t (&acc)[NS][NBW], const f
   // the loop); their results are never stored
   int rb[NBW];
 #pragma unroll
-  for (int i = 0; i < NBW; ++i) rb[i] = (min(wv + NW * i, nblk - 1) * 16 + kk) * kMfS + ii;
-  for (int d = 0; d < nd; ++d) {
+  for (int i = 0; i < NBW; ++i) rb[i] = (min(wv + NW * i, nblk - 1) * 16 + kk) * ps + ii * ls;
+  // Software-pipelined over two operand sets: the operands of step st + 1
+  // are read from LDS while the products of step st run, so no product
+  // waits for its own reads (a read past the last step re-reads the last
+  // step's operands, which are then unused).
+  const int w0 = 15 + kk - ii, last = nsteps - 1;
+  auto fetch = [&](float (&a)[NS], float (&b)[NBW], int st) {
+    const int t = min(st, last);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (4 * d + u >= nsteps) break;
-      const int wi = 15 + 16 * d + 4 * u + kk - ii;
-      float av[NS];
+    for (int s = 0; s < NS; ++s) a[s] = wp[s][w0 + 4 * t];
 #pragma unroll
-      for (int s = 0; s < NS; ++s) av[s] = wp[s][wi];
+    for (int i = 0; i < NBW; ++i) b[i] = X[rb[i] + 4 * t * ps];
+  };
+  auto step = [&](const float (&a)[NS], const float (&b)[NBW]) {
 #pragma unroll
-      for (int i = 0; i < NBW; ++i) {
-        const float bx = X[rb[i] + (16 * d + 4 * u) * kMfS];
+    for (int i = 0; i < NBW; ++i) {
 #pragma unroll
-        for (int s = 0; s < NS; ++s) acc[s][i] = mfma4(av[s], bx, acc[s][i]);
-      }
+      for (int s = 0; s < NS; ++s) acc[s][i] = mfma4(a[s], b[i], acc[s][i]);
     }
+  };
+  float a0[NS], b0[NBW], a1[NS], b1[NBW];
+  fetch(a0, b0, 0);
+  int st = 0;
+  for (; st + 1 < nsteps; st += 2) {
+    fetch(a1, b1, st + 1);
+    step(a0, b0);
+    fetch(a0, b0, st + 2);
+    step(a1, b1);
   }
+  if (st < nsteps) step(a0, b0);  // an odd step count's last step
+  (void)nd;
 }
 
 // The reference's arithmetic for the same outputs, in the same layout (the
@@ -320,7 +345,7 @@ __device__ __forceinline__ void chain_pass_mfma(f32x4_t (&acc)[NS][NBW], const f
 template <int NBW, int NS, int NW>
 __device__ __forceinline__ void chain_pass_exact(f32x4_t (&acc)[NS][NBW], const float* X,
                                                  const float* const (&wp)[NS], int Km,
-                                                 int nblk, int wv, int lane) {
+                                                 int nblk, int wv, int lane, int ps, int ls) {
   const int kk = lane >> 4, ii = lane & 15;
 #pragma unroll
   for (int i = 0; i < NBW; ++i) {
@@ -336,7 +361,7 @@ __device__ __forceinline__ void chain_pass_exact(f32x4_t (&acc)[NS][NBW], const 
         float v = 0.f;
         for (int t = 0; t < Km; ++t) {
           const float w = wp[s][15 + t];
-          if (w != 0.f) v = fmaf(X[(p + t) * kMfS + ii], w, v);
+          if (w != 0.f) v = fmaf(X[(p + t) * ps + ii * ls], w, v);
         }
         acc[s][i][r] = v;
       }
@@ -347,11 +372,12 @@ __device__ __forceinline__ void chain_pass_exact(f32x4_t (&acc)[NS][NBW], const 
 template <int NBW, int NS, int NW>
 __device__ __forceinline__ void chain_pass(bool exact, f32x4_t (&acc)[NS][NBW], const float* X,
                                            const float* const (&wp)[NS], int Km, int nd,
-                                           int nblk, int wv, int lane) {
+                                           int nblk, int wv, int lane, int ps = kMfS,
+                                           int ls = 1) {
   if (exact)
-    chain_pass_exact<NBW, NS, NW>(acc, X, wp, Km, nblk, wv, lane);
+    chain_pass_exact<NBW, NS, NW>(acc, X, wp, Km, nblk, wv, lane, ps, ls);
   else
-    chain_pass_mfma<NBW, NS, NW>(acc, X, wp, nd, nblk, wv, lane, Km);
+    chain_pass_mfma<NBW, NS, NW>(acc, X, wp, nd, nblk, wv, lane, Km, ps, ls);
 }
 
 __device__ __forceinline__ float load_in(const void* in, int dt, int64_t g, bool negate,
@@ -387,14 +413,15 @@ __device__ __forceinline__ void halo_table(short* hsrc, int E, int64_t N, int sh
 
 // Boundary-extend the halo rows of X (and Y) from the rows inside.
 __device__ __forceinline__ void fill_halo(float* X, float* Y, int E, int N, int shift,
-                                          const short* hsrc, float cval, int tid, int nt) {
+                                          const short* hsrc, float cval, int tid, int nt,
+                                          int ps = kMfS, int ls = 1) {
   const int nh = E - N;
   for (int e = tid; e < nh * kChC; e += nt) {
     const int h = e >> 4, c = e & 15;
     const int q = h < shift ? h : h + N;
     const int src = hsrc[h];
-    X[q * kMfS + c] = src >= 0 ? X[src * kMfS + c] : cval;
-    if (Y) Y[q * kMfS + c] = src >= 0 ? Y[src * kMfS + c] : cval;
+    X[q * ps + c * ls] = src >= 0 ? X[src * ps + c * ls] : cval;
+    if (Y) Y[q * ps + c * ls] = src >= 0 ? Y[src * ps + c * ls] : cval;
   }
 }
 
@@ -713,6 +740,7 @@ struct FinalArgs {
   void* rg_max;     // uint32 (float) / uint64 (double) bit pattern
   int t0, Km, shift;  // matrix-core variant: nonzero tap span of S and D,
                       // buffer row of line position 0
+  int vec4;           // matrix-core variant: rows staged by 16-byte loads
 };
 
 template <typename W>
@@ -893,79 +921,103 @@ __global__ __launch_bounds__(256) void k_final_contig(const FinalArgs<W> a) {
 // the one of k_final_contig.  The tile's next input is in flight in
 // registers while the current one is computed.
 struct FinalGeom {
-  int E, Km, shift, nd, nblk;
+  int E, Km, shift, nd, nblk, LS;
   int64_t N, nrows, ntile, G;
 };
 
-// Thread's share of a tile's interior: row c = u % 16, position
-// tid + 64 NW (u / 16).
-template <int NPF, int NW>
-__device__ __forceinline__ void final_load(const FinalArgs<float>& a, const FinalGeom& g,
-                                           float (&v)[NPF], int J, int64_t tile, int tid) {
-  const int64_t row0 = tile * kChC;
-  const int n = (int)g.N;
+// Stage input J's rows of the tile into the line-major buffer X (row r of
+// line l at X[l * LS + r], interior from row shift): thread t takes line
+// l = t % 16 and the 4-value chunks c = t / 16 + (NT / 16) k, as 16-byte
+// loads and two 8-byte stores when vec (rows of a multiple of 4 values,
+// 16-byte aligned), else value by value.  Rows past the last are staged as
+// zeros.  Every load is issued before any is used (the index of a masked
+// one is clamped into the row and its value replaced afterwards).  Returns
+// whether a staged value is not finite.
+template <int NW>
+__device__ __forceinline__ bool final_stage(const FinalArgs<float>& a, const FinalGeom& g,
+                                            float* X, int LS, int J, int64_t tile, int tid) {
+  typedef float F4 __attribute__((ext_vector_type(4)));
+  typedef float F2 __attribute__((ext_vector_type(2)));
+  constexpr int CS = 4 * NW;  // chunk slots per pass over the work-group
+  const int l = tid & 15, c0 = tid >> 4, n = (int)g.N;
+  const int64_t row = tile * kChC + l;
+  const bool rok = row < g.nrows;
+  const float* rp = a.in[J] + (rok ? row : 0) * g.N;
+  float* xl = X + l * LS + g.shift;
+  bool bad = false;
+  if (a.vec4) {
+    // N <= 64 NW, so a line has at most 4 CS chunks
+    F4 v[4];
 #pragma unroll
-  for (int u = 0; u < NPF; ++u) {
-    // a wave-uniform row pointer and a 32-bit lane offset
-    const int64_t row = row0 + (u & 15);
-    const float* rp = a.in[J] + row * g.N;
-    const int q = tid + 64 * NW * (u >> 4);
-    v[u] = (row < g.nrows && q < n) ? rp[q] : 0.f;
+    for (int k = 0; k < 4; ++k) {
+      const int c = c0 + CS * k;
+      v[k] = reinterpret_cast<const F4*>(rp)[4 * c < n ? c : 0];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int c = c0 + CS * k;
+      if (4 * c >= n) continue;
+      const F4 w = rok ? v[k] : F4{0.f, 0.f, 0.f, 0.f};
+      bad |= !(finite(w[0]) && finite(w[1]) && finite(w[2]) && finite(w[3]));
+      *reinterpret_cast<F2*>(xl + 4 * c) = F2{w[0], w[1]};
+      *reinterpret_cast<F2*>(xl + 4 * c + 2) = F2{w[2], w[3]};
+    }
+  } else {
+    float v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int q = c0 + CS * k;
+      v[k] = rp[q < n ? q : 0];
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int q = c0 + CS * k;
+      if (q >= n) continue;
+      const float w = rok ? v[k] : 0.f;
+      bad |= !finite(w);
+      xl[q] = w;
+    }
   }
+  return bad;
 }
 
-// Whether the final pass loads input J + 1 into registers while input J is
-// computed (more registers per lane, fewer resident waves).
-constexpr bool kFinalPrefetch = false;
-
-// Input J of the tile: store the loaded input, start the next load, first
-// op, re-extend, second op into res.
+// Input J of the tile: stage it, first op, re-extend, second op into res.
 template <int NIN, int NBW, int NW, int J>
 __device__ __forceinline__ void final_input_mfma(const FinalArgs<float>& a, const FinalGeom& g,
                                                  float* xin, float* xf, const float* wpS,
                                                  const float* wpD, const short* hsrc,
-                                                 int64_t tile, float (&pf)[kMfmaNpf],
-                                                 f32x4_t (&res)[NBW]) {
-  const int E = g.E, shift = g.shift;
+                                                 int64_t tile, f32x4_t (&res)[NBW]) {
+  typedef float F2 __attribute__((ext_vector_type(2)));
+  const int E = g.E, shift = g.shift, LS = g.LS;
   const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
   const int kk = lane >> 4, ii = lane & 15;
-  constexpr int NPF = kMfmaNpf, NT = 64 * NW;
-  if (!kFinalPrefetch) final_load<NPF, NW>(a, g, pf, J, tile, tid);
-  bool bad = false;
-#pragma unroll
-  for (int u = 0; u < NPF; ++u) {
-    const int q = tid + NT * (u >> 4);
-    if (q < g.N) {
-      xin[(shift + q) * kMfS + (u & 15)] = pf[u];
-      bad |= !finite(pf[u]);
-    }
-  }
+  constexpr int NT = 64 * NW;
+  const bool bad = final_stage<NW>(a, g, xin, LS, J, tile, tid);
   const bool skip0 = __syncthreads_or(bad) || !finite(a.cval);
-  if (kFinalPrefetch && J + 1 < NIN)
-    final_load<NPF, NW>(a, g, pf, J + 1 < NIN ? J + 1 : 0, tile, tid);
-  fill_halo(xin, nullptr, E, (int)g.N, shift, hsrc, a.cval, tid, NT);
+  fill_halo(xin, nullptr, E, (int)g.N, shift, hsrc, a.cval, tid, NT, 1, LS);
   __syncthreads();
   f32x4_t f[1][NBW];
   {
     const float* const wp[1] = {a.first[J] ? wpD : wpS};
-    chain_pass<NBW, 1, NW>(skip0, f, xin, wp, g.Km, g.nd, g.nblk, wv, lane);
+    chain_pass<NBW, 1, NW>(skip0, f, xin, wp, g.Km, g.nd, g.nblk, wv, lane, 1, LS);
   }
   bool bad1 = false;
 #pragma unroll
   for (int i = 0; i < NBW; ++i) {
     if (wv + NW * i >= g.nblk) continue;
+    // positions 16 b + 4 kk + r (r < 4) of line ii: two 8-byte words
+    float* d = xf + ii * LS + shift + 16 * (wv + NW * i) + 4 * kk;
+    *reinterpret_cast<F2*>(d) = F2{f[0][i][0], f[0][i][1]};
+    *reinterpret_cast<F2*>(d + 2) = F2{f[0][i][2], f[0][i][3]};
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      xf[(shift + 16 * (wv + NW * i) + 4 * kk + r) * kMfS + ii] = f[0][i][r];
-      bad1 |= !finite(f[0][i][r]);
-    }
+    for (int r = 0; r < 4; ++r) bad1 |= !finite(f[0][i][r]);
   }
   const bool skip1 = __syncthreads_or(bad1) || skip0;
-  fill_halo(xf, nullptr, E, (int)g.N, shift, hsrc, a.cval, tid, NT);
+  fill_halo(xf, nullptr, E, (int)g.N, shift, hsrc, a.cval, tid, NT, 1, LS);
   __syncthreads();
   {
     const float* const wp[1] = {a.second[J] ? wpD : wpS};
-    chain_pass<NBW, 1, NW>(skip1, f, xf, wp, g.Km, g.nd, g.nblk, wv, lane);
+    chain_pass<NBW, 1, NW>(skip1, f, xf, wp, g.Km, g.nd, g.nblk, wv, lane, 1, LS);
   }
 #pragma unroll
   for (int i = 0; i < NBW; ++i) res[i] = f[0][i];
@@ -978,11 +1030,9 @@ template <int NIN, int NBW, int NW, size_t... Js>
 __device__ __forceinline__ void final_all_mfma(const FinalArgs<float>& a, const FinalGeom& g,
                                                float* xin, float* xf, const float* wpS,
                                                const float* wpD, const short* hsrc,
-                                               int64_t tile, float (&pf)[kMfmaNpf],
-                                               f32x4_t (&res)[NIN][NBW],
+                                               int64_t tile, f32x4_t (&res)[NIN][NBW],
                                                std::index_sequence<Js...>) {
-  (final_input_mfma<NIN, NBW, NW, (int)Js>(a, g, xin, xf, wpS, wpD, hsrc, tile, pf, res[Js]),
-   ...);
+  (final_input_mfma<NIN, NBW, NW, (int)Js>(a, g, xin, xf, wpS, wpD, hsrc, tile, res[Js]), ...);
 }
 
 // EP: the epilogue, fixed at compile time so that each variant keeps only
@@ -992,7 +1042,7 @@ __device__ __forceinline__ void final_all_mfma(const FinalArgs<float>& a, const 
 template <int NIN, int NBW, int NW, int EP>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void k_final_mfma(
     const FinalArgs<float> a) {
-  constexpr int NPF = kMfmaNpf, NT = 64 * NW;
+  constexpr int NT = 64 * NW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   FinalGeom g;
   g.E = a.E;
@@ -1000,14 +1050,18 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
   g.shift = a.shift;
   g.nd = (a.Km + 30) / 16;
   g.nblk = a.Nr / 16;
+  g.LS = lm_stride(a.E);
   g.N = a.N;
   g.nrows = a.nrows;
   g.ntile = (a.nrows + kChC - 1) / kChC;
   g.G = gridDim.x;
   const int WP = mfma_wp(g.Km);
-  float* xin = reinterpret_cast<float*>(smem_raw);
-  float* xf = xin + g.E * kMfS;
-  float* wpS = xf + g.E * kMfS;
+  // two line-major buffers of 16 lines; the interior of a line (row shift)
+  // starts at an even word, so that its 8-byte words are aligned
+  float* smem_f = reinterpret_cast<float*>(smem_raw);
+  float* xin = smem_f + (g.shift & 1);
+  float* xf = xin + kChC * g.LS;
+  float* wpS = smem_f + 2 + 2 * kChC * g.LS;
   float* wpD = wpS + WP;
   short* hsrc = reinterpret_cast<short*>(wpD + WP);
   const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
@@ -1019,13 +1073,11 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
   const FrangiNegRecip nrc = frangi_neg_recip(a.alpha_sq, a.beta_sq, gsq);
   const int64_t N = g.N;
   float rgmax = 0.f;
-  float pf[NPF];
   const int64_t tile = xcd_swizzle(blockIdx.x, g.G);
-  if (kFinalPrefetch) final_load<NPF, NW>(a, g, pf, 0, tile, tid);
   {
     f32x4_t res[NIN][NBW];
-    final_all_mfma<NIN, NBW, NW>(a, g, xin, xf, wpS, wpD, hsrc, tile, pf, res,
-                             std::make_index_sequence<NIN>{});
+    final_all_mfma<NIN, NBW, NW>(a, g, xin, xf, wpS, wpD, hsrc, tile, res,
+                                 std::make_index_sequence<NIN>{});
     // epilogue (k_final_contig's)
     const int64_t row = tile * kChC + ii;
     if (row < g.nrows) {
@@ -1235,7 +1287,7 @@ struct MfmaGeomHost {
 };
 
 inline bool mfma_geom(const TapPack<float>& S, const TapPack<float>& D, int64_t N, int nbuf,
-                      MfmaGeomHost& g) {
+                      MfmaGeomHost& g, bool line_major = false) {
   if (!chain_mfma_enabled() || N < 1 || N > 512) return false;
   int lo = S.K, hi = -1;
   for (int t = 0; t < S.K; ++t)
@@ -1255,10 +1307,11 @@ inline bool mfma_geom(const TapPack<float>& S, const TapPack<float>& D, int64_t 
   const int nsteps = (g.Km + 18) / 4;
   g.E = std::max(std::max(g.Nr + 4 * nsteps - 16, g.Nr + g.Km - 1), g.shift + g.Nr);
   g.nd = (g.Km + 30) / 16;
-  // nbuf line buffers, the padded taps, the halo source rows (16-bit, one
+  // nbuf line buffers (line-major: 16 lines of lm_stride(E) and two words
+  // of alignment room), the padded taps, the halo source rows (16-bit, one
   // per row outside [shift, shift + N))
-  g.lds = (nbuf * g.E * kMfS + 2 * mfma_wp(g.Km)) * (int)sizeof(float) +
-          ((int)(g.E - N) + 1) / 2 * 4;
+  const int buf = line_major ? 2 + nbuf * kChC * lm_stride(g.E) : nbuf * g.E * kMfS;
+  g.lds = (buf + 2 * mfma_wp(g.Km)) * (int)sizeof(float) + ((int)(g.E - N) + 1) / 2 * 4;
   return g.lds <= kMaxLds;
 }
 
@@ -1400,13 +1453,16 @@ template <int NIN>
 int launch_final_mfma(const FinalArgs<float>& a0, hipStream_t st) {
   const int nblk = (int)((a0.N + 15) / 16);
   MfmaGeomHost g;
-  if (!mfma_geom(a0.S, a0.D, a0.N, 2, g)) return -1;
+  if (!mfma_geom(a0.S, a0.D, a0.N, 2, g, true)) return -1;
   FinalArgs<float> a = a0;
   a.t0 = g.t0;
   a.Km = g.Km;
   a.shift = g.shift;
   a.Nr = g.Nr;
   a.E = g.E;
+  // rows of a multiple of 4 values from 16-byte aligned arrays: 16-byte loads
+  a.vec4 = a.N % 4 == 0;
+  for (int j = 0; j < NIN; ++j) a.vec4 &= ((uintptr_t)a.in[j] & 15) == 0;
   if (mfma_waves(a.N) == 8) return launch_final_mfma_nbw<NIN, 4, 8>(a, g, st);
   if (nblk <= 4) return launch_final_mfma_nbw<NIN, 1, 4>(a, g, st);
   if (nblk <= 8) return launch_final_mfma_nbw<NIN, 2, 4>(a, g, st);
