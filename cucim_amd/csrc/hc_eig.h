@@ -45,27 +45,35 @@ __device__ __forceinline__ void eig22(F M00, F M01, F M11, int descending, int a
   }
 }
 
-// cos(x) for |x| <= 2.1, in double: the Taylor series of cos to the x^26
-// term (truncation below 1e-18 on that range), by Horner's rule in x^2.  The
-// three angles of the closed form below, (phi + k pi) / 3 with phi in
-// [0, pi], lie in [-pi/3, 2 pi/3], so the general argument reduction of the
-// library cos is not needed.
-__device__ __forceinline__ double cos_small(double x) {
-  const double z = x * x;
-  double p = -1.0 / 403291461126605635584000000.0;  // -1/26!
-  p = fma(p, z, 1.0 / 620448401733239439360000.0);  // 1/24!
-  p = fma(p, z, -1.0 / 1124000727777607680000.0);   // -1/22!
-  p = fma(p, z, 1.0 / 2432902008176640000.0);       // 1/20!
-  p = fma(p, z, -1.0 / 6402373705728000.0);         // -1/18!
-  p = fma(p, z, 1.0 / 20922789888000.0);            // 1/16!
-  p = fma(p, z, -1.0 / 87178291200.0);              // -1/14!
-  p = fma(p, z, 1.0 / 479001600.0);                 // 1/12!
-  p = fma(p, z, -1.0 / 3628800.0);                  // -1/10!
-  p = fma(p, z, 1.0 / 40320.0);                     // 1/8!
-  p = fma(p, z, -1.0 / 720.0);                      // -1/6!
-  p = fma(p, z, 1.0 / 24.0);                        // 1/4!
-  p = fma(p, z, -0.5);                              // -1/2!
-  return fma(p, z, 1.0);
+// cos(t) and sin(t) for t in [0, pi/3], in double: the Taylor series to the
+// t^18 and t^19 terms (truncation below 2e-18 on that range), by Horner's
+// rule in t^2.  The closed form below needs the cosines of (phi + k pi) / 3
+// with phi in [0, pi]; with t = phi / 3 they are cos t and
+// cos t / 2 -+ (sqrt(3) / 2) sin t, so one pair on [0, pi/3] serves all
+// three and the general argument reduction of the library functions is not
+// needed.
+__device__ __forceinline__ void sincos_small(double t, double& c, double& s) {
+  const double z = t * t;
+  double pc = 1.0 / 6402373705728000.0;  // 1/18!
+  pc = fma(pc, z, -1.0 / 20922789888000.0);  // -1/16!
+  pc = fma(pc, z, 1.0 / 87178291200.0);      // 1/14!
+  pc = fma(pc, z, -1.0 / 479001600.0);       // -1/12!
+  pc = fma(pc, z, 1.0 / 3628800.0);          // 1/10!
+  pc = fma(pc, z, -1.0 / 40320.0);           // -1/8!
+  pc = fma(pc, z, 1.0 / 720.0);              // 1/6!
+  pc = fma(pc, z, -1.0 / 24.0);              // -1/4!
+  pc = fma(pc, z, 0.5);                      // 1/2!
+  c = fma(-pc, z, 1.0);
+  double ps = -1.0 / 121645100408832000.0;   // -1/19!
+  ps = fma(ps, z, 1.0 / 355687428096000.0);  // 1/17!
+  ps = fma(ps, z, -1.0 / 1307674368000.0);   // -1/15!
+  ps = fma(ps, z, 1.0 / 6227020800.0);       // 1/13!
+  ps = fma(ps, z, -1.0 / 39916800.0);        // -1/11!
+  ps = fma(ps, z, 1.0 / 362880.0);           // 1/9!
+  ps = fma(ps, z, -1.0 / 5040.0);            // -1/7!
+  ps = fma(ps, z, 1.0 / 120.0);              // 1/5!
+  ps = fma(ps, z, -1.0 / 6.0);               // -1/3!
+  s = fma(ps * z, t, t);
 }
 
 // 3x3, Deledalle et al. (2017) closed form in double (corner.py:566-612) for
@@ -100,9 +108,13 @@ __device__ __forceinline__ void eig33(F aa, F bb, F cc, F dd, F ee, F ff, int de
   constexpr double third = 1.0 / 3.0;
   const double x1_term = (2.0 / 3.0) * sqrt(x1);
   const double abc = (a + b + c) * third;
-  F lam1 = (F)(abc - x1_term * cos_small(phi * third));
-  F lam2 = (F)(abc + x1_term * cos_small((phi - M_PI) * third));
-  F lam3 = (F)(abc + x1_term * cos_small((phi + M_PI) * third));
+  // cos(phi / 3), cos((phi - pi) / 3), cos((phi + pi) / 3)
+  double ct, st;
+  sincos_small(phi * third, ct, st);
+  const double half_c = 0.5 * ct, r3_s = 0.86602540378443864676 * st;  // sqrt(3) / 2
+  F lam1 = (F)(abc - x1_term * ct);
+  F lam2 = (F)(abc + x1_term * (half_c + r3_s));
+  F lam3 = (F)(abc + x1_term * (half_c - r3_s));
   // the network runs on (var1, lam2, var3) = (lam1, lam2, lam3) ascending,
   // (lam3, lam2, lam1) descending
   F& v1 = descending ? lam3 : lam1;
