@@ -25,7 +25,10 @@ OBJ = os.path.join(ROOT, "_build")
 SOURCES = ["hc_host.hip", "k_correlate.hip", "k_hessian.hip", "k_pointwise.hip",
            "k_stencil2d.hip", "k_ridges.hip", "k_cnn3d.hip",
            "k_frangi_check.hip", "k_eig_nd.hip", "k_measure.hip",
-           "hc_taps.hip"]
+           "hc_taps.hip", "hc_wsi.hip"]
+
+# system libraries the library links (zlib: deflate-compressed TIFF tiles)
+LIBS = ["-lz"]
 
 ARCH = "gfx950"
 # Packed FP32 instructions are disabled for every HIP translation unit.
@@ -63,7 +66,7 @@ def build(verbose=False):
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
+    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, *LIBS, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
@@ -110,7 +113,7 @@ def build_host_sanitized(driver, outdir):
         raise RuntimeError(f"clang++ failed for {driver}:\n{r.stderr}")
     exe = os.path.join(outdir, stem)
     r = subprocess.run([_hipcc(), "-fgpu-rdc", "--hip-link", f"--offload-arch={ARCH}",
-                        "-fsanitize=address,undefined", *objs, dobj, "-o", exe],
+                        "-fsanitize=address,undefined", *objs, dobj, *LIBS, "-o", exe],
                        capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link (sanitized) failed:\n{r.stderr}")

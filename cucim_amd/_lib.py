@@ -68,6 +68,13 @@ ENTRY_POINTS = (
     "hc_li_means",
     "hc_label_work_elems",
     "hc_label",
+    "hc_wsi_open",
+    "hc_wsi_close",
+    "hc_wsi_level_count",
+    "hc_wsi_level_info",
+    "hc_wsi_properties",
+    "hc_wsi_read_region",
+    "hc_wsi_read_region_device",
     "hc_sqrt_inplace",
     "hc_copy_cast",
     "hc_batch2d_plan",
@@ -157,6 +164,15 @@ _SIGNATURES = {
     "hc_li_means": ([_A, _d, _d, _v, _v, ctypes.c_int64, _v], _i),
     "hc_label_work_elems": ([ctypes.c_int64], ctypes.c_int64),
     "hc_label": ([_A, ctypes.c_int64, _i, _A, _v, ctypes.c_int64, _v, _v], _i),
+    "hc_wsi_open": ([ctypes.c_char_p, _P(_v)], _i),
+    "hc_wsi_close": ([_v], None),
+    "hc_wsi_level_count": ([_v], _i),
+    "hc_wsi_level_info": ([_v, _i, _P(ctypes.c_int64)], _i),
+    "hc_wsi_properties": ([_v, _P(_d), _P(_i), ctypes.c_char_p, _i, ctypes.c_char_p, _i], _i),
+    "hc_wsi_read_region": ([_v, _i, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                            ctypes.c_int64, _v, _i], _i),
+    "hc_wsi_read_region_device": ([_v, _i, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                   ctypes.c_int64, _v, _i, _v], _i),
     "hc_sqrt_inplace": ([_A, _v], _i),
     "hc_copy_cast": ([_A, _A, _v], _i),
     "hc_batch2d_plan": ([_v, _i, _i, _i, _v, ctypes.c_int64], ctypes.c_int64),

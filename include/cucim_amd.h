@@ -461,6 +461,45 @@ int64_t hc_label_work_elems(int64_t n);
 int hc_label(const hc_array* in, int64_t background, int connectivity, const hc_array* out,
              int32_t* work_dev, int64_t work_elems, int32_t* count_dev, void* stream);
 
+/* ---- before the filters: whole-slide image tiles ---- */
+
+/* An open TIFF / BigTIFF (pyramid) file: the reference's CuImage over the
+ * cuslide plugin (cpp/include/cucim/cuimage.h:168, cpp/src/cuimage.cpp,
+ * cpp/plugins/cucim.kit.cuslide/src/cuslide/tiff/tiff.cpp, ifd.cpp).
+ * Levels: IFD 0 and every following tiled IFD smaller than the level before
+ * it.  Codecs: none, LZW, deflate, baseline JPEG (TIFF JPEGTables
+ * supported); 8-bit interleaved samples; horizontal predictor. */
+typedef struct hc_wsi hc_wsi;
+int hc_wsi_open(const char* path, hc_wsi** out);
+void hc_wsi_close(hc_wsi* h);
+int hc_wsi_level_count(const hc_wsi* h);
+/* info[8]: width, height, tile width, tile height (strip: width x rows per
+ * strip), output samples per pixel (JPEG: 3 or 1), compression, photometric,
+ * tiled flag. */
+int hc_wsi_level_info(const hc_wsi* h, int level, int64_t* info);
+/* res[3]: x resolution, y resolution, resolution unit (TIFF tags 282, 283,
+ * 296 of IFD 0); background: the fill value of pixels outside the image
+ * (255 for Aperio / Philips files, else 0, ifd.cpp:605); desc: the image
+ * description of IFD 0; vendor: "aperio", "philips" or "generic".  Any
+ * pointer may be NULL. */
+int hc_wsi_properties(const hc_wsi* h, double* res, int* background, char* desc, int desc_len,
+                      char* vendor, int vendor_len);
+/* read_region (cuimage.h:168, ifd.cpp:576-1260): the region [x, x + w) x
+ * [y, y + h) of a level (location may be negative or reach past the edge;
+ * outside pixels take the background value) as h x w x channels uint8 into
+ * host memory, tiles decoded on num_workers host threads (0 or 1: the
+ * calling thread). */
+int hc_wsi_read_region(const hc_wsi* h, int level, int64_t x, int64_t y, int64_t w, int64_t hgt,
+                       uint8_t* out, int num_workers);
+/* The same region into device memory: JPEG tiles are uploaded compressed
+ * and decoded on the GPU (entropy decoding one work-group per tile from LDS,
+ * integer inverse DCT, fancy chroma upsampling and YCbCr -> RGB in device
+ * kernels, bit-exact to the host path and to libjpeg-turbo); lossless tiles
+ * are decoded on host threads and uploaded.  Returns after the region is
+ * complete on the stream. */
+int hc_wsi_read_region_device(const hc_wsi* h, int level, int64_t x, int64_t y, int64_t w,
+                              int64_t hgt, uint8_t* out_dev, int num_workers, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

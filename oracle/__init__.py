@@ -28,4 +28,8 @@ How each function is pinned (independent checks, run by ``tests/test_oracle_*``)
 * ``skimage.frangi``             -- ``filters/tests/test_ridges.py`` invariants
                                     (null image, black/white symmetry, linearity)
 * ``skimage.sobel``              -- ``filters/tests/test_edges.py:72-106`` known answers
+* ``wsi.TiffFile.read_region``   -- files libtiff wrote (lossless: the array written;
+                                    JPEG: libtiff's own decode); tiles decoded by
+                                    libjpeg-turbo / zlib, the libraries the reference
+                                    links; ``wsi.lzw_decode`` by libtiff-read round trips
 """
