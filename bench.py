@@ -43,7 +43,7 @@ def _parse():
     p.add_argument("--parity-edge", type=int, default=96,
                    help="edge of the cubic crop compared against the FP64 oracle")
     p.add_argument("--only", choices=["all", "frangi", "correlate", "copy", "batch2d",
-                                      "surrogate", "batch3d", "vesselness"],
+                                      "surrogate", "batch3d", "vesselness", "wsi"],
                    default="all",
                    help="restrict the run (profiler passes); 'copy' runs the plain copy "
                         "kernel that calibrates the HBM byte counters")
@@ -640,6 +640,76 @@ def bench_vesselness(reps=5, cpu=True):
     return res
 
 
+def bench_wsi(reps=5, cpu=True):
+    """Whole-slide region read (the reference's CuImage.read_region): a
+    synthetic 4096 x 4096 RGB pyramid (3 levels, 256 x 256 baseline-JPEG tiles,
+    quality 90, 4:2:0, shared JPEGTables) written to a temporary file; the
+    whole level 0 read into device memory (JPEG tiles decoded by the device
+    kernels), alone and followed by rgb2gray + gaussian(sigma=2); the host
+    decode path on the host threads; the CPU baseline is the oracle's
+    libjpeg-turbo decode of every tile (one thread)."""
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from cucim_amd import _lib
+    from cucim_amd.clara import CuImage
+    from cucim_amd.skimage import color, filters
+    from tests.wsi_tiff import pyramid, write_tiff
+
+    edge = 4096
+    res = {"image": [edge, edge, 3], "tile": [256, 256], "codec": "jpeg q90 4:2:0"}
+    with tempfile.TemporaryDirectory() as d:
+        path = write_tiff(os.path.join(d, "slide.tif"), pyramid(edge, edge, 3, seed=11),
+                          tile=(256, 256), compression="jpeg", quality=90, subsampling=2)
+        res["file_mb"] = os.path.getsize(path) / 1e6
+        im = CuImage(path)
+        out = torch.empty((edge, edge, 3), dtype=torch.uint8, device="cuda")
+
+        def timed(fn, n):
+            fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(n):
+                fn()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / n
+
+        rd = lambda: im.read_region((0, 0), (edge, edge), 0, device="cuda", buf=out)  # noqa: E731
+        t = timed(rd, reps)
+        res["device_read_ms"] = t * 1e3
+        res["device_read_mpx_s"] = edge * edge / t / 1e6
+        with _lib.KernelTimer() as kt:
+            rd()
+            torch.cuda.synchronize()
+        res["kernels_ms"] = {k: v for k, v in kt.collect()}
+        pipe = lambda: filters.gaussian(color.rgb2gray(rd()), sigma=2.0)  # noqa: E731
+        t = timed(pipe, reps)
+        res["read_gray_gaussian_ms"] = t * 1e3
+        nw = min(16, os.cpu_count() or 1)
+        hb = np.empty((edge, edge, 3), np.uint8)
+        im.read_region((0, 0), (edge, edge), 0, num_workers=nw, buf=hb)
+        t0 = time.perf_counter()
+        im.read_region((0, 0), (edge, edge), 0, num_workers=nw, buf=hb)
+        th = time.perf_counter() - t0
+        res["host_read"] = {"ms": th * 1e3, "mpx_s": edge * edge / th / 1e6, "threads": nw}
+        res["device_equals_host"] = bool(np.array_equal(out.cpu().numpy(), hb))
+        if cpu:
+            from oracle.wsi import TiffFile
+
+            o = TiffFile(path)
+            t0 = time.perf_counter()
+            o.level_image(0)
+            tc = time.perf_counter() - t0
+            res["cpu_baseline"] = {"value": edge * edge / tc / 1e6, "unit": "Mpx/s",
+                                   "cores": 1, "kind": "port",
+                                   "sample": "every level-0 tile decoded by libjpeg-turbo "
+                                             "(Pillow) and assembled, one thread"}
+        im.close()
+    return res
+
+
 def _corr_roofline(corr):
     """HBM roofline of the correlate1d line kernel (contiguous axis, 8192^2
     float32, K = 17): algorithmic bytes (in + out) per launch over its
@@ -686,6 +756,9 @@ def main():
         print(json.dumps({"vesselness": bench_vesselness(max(2, args.steps // 4),
                                                          not args.no_cpu)}))
         return
+    if args.only == "wsi":
+        print(json.dumps({"wsi_read": bench_wsi(max(2, args.steps // 4), not args.no_cpu)}))
+        return
     if args.only == "correlate":
         sizes = tuple(int(v) for v in args.corr_sizes.split(","))
         print(json.dumps({"copy": bench_copy(), "correlate1d": bench_correlate(sizes)}))
@@ -711,9 +784,10 @@ def main():
     corr = None
     if args.only == "all":
         corr = bench_correlate(tuple(int(v) for v in args.corr_sizes.split(",")))
-    batch2d = surrogate = batch3d = vessel = None
+    batch2d = surrogate = batch3d = vessel = wsi = None
     if args.only == "all":
         vessel = bench_vesselness(max(2, args.steps // 4), not args.no_cpu)
+        wsi = bench_wsi(max(2, args.steps // 4), not args.no_cpu)
         batch2d = bench_batch2d(args.steps, 0 if args.no_cpu else args.cpu_tiles)
         batch3d = bench_batch3d(max(2, args.steps // 4))
         surrogate = bench_surrogate(max(2, args.steps // 4))
@@ -765,6 +839,7 @@ def main():
         "cpu_baseline": cpu,
         "parity": parity,
         "vesselness_panel": vessel,
+        "wsi_read": wsi,
         "batch2d": batch2d,
         "batch3d": batch3d,
         "surrogate": surrogate,

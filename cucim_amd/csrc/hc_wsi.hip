@@ -650,9 +650,9 @@ __host__ __device__ void jpeg_entropy(const JTile& jt, const uint8_t* data, int1
               continue;
             }
             i += r;
-            if (i > 63) break;
-            const int z = kZigzag[i];
-            blk[z] = (int16_t)extend(br.get(sz), sz);
+            // libjpeg's natural-order table maps the positions past 63 of a
+            // corrupt stream to 63; the loop then ends
+            blk[kZigzag[i < 63 ? i : 63]] = (int16_t)extend(br.get(sz), sz);
             ++i;
           }
         }
@@ -883,6 +883,95 @@ __device__ inline int huff_lds(LdsBits& br, const HuffTab& t) {
   return t.vals[t.valptr[l] + code - t.mincode[l]];
 }
 
+// Decoder state at a codeword boundary: bit position, unit of the MCU (the
+// block within it) and coefficient index (0: the DC difference is next).
+struct JState {
+  int64_t bit;
+  int u, z;
+};
+
+// What a lane needs to decode part of one tile's destuffed segment.
+struct JSeg {
+  const uint8_t* s;
+  int n;
+  const HuffTab* tabs;  // dc0, dc1, ac0, ac1
+  const uint8_t* zz;
+  const int8_t *ucomp, *ubx, *uby;
+  int U, mcux;
+  int total;            // blocks of the tile
+  int16_t* coef;
+  int dtab[3], atab[3], ch[3], cv[3], cbw[3];
+  int64_t cb0[3];
+};
+
+__device__ inline int16_t* jseg_block(const JSeg& sg, int g, int u) {
+  const int m = g / sg.U;
+  const int c = sg.ucomp[u];
+  const int my = m / sg.mcux, mx = m - my * sg.mcux;
+  const int64_t bidx =
+      (int64_t)(my * sg.cv[c] + sg.uby[u]) * sg.cbw[c] + (mx * sg.ch[c] + sg.ubx[u]);
+  return sg.coef + (sg.cb0[c] + bidx) * 64;
+}
+
+// DC coefficient of the k-th block of component c in decode order.
+__device__ inline int16_t* jseg_dc(const JSeg& sg, int c, int hv, int k) {
+  const int m = k / hv;
+  const int r = k - m * hv;
+  const int by = r / sg.ch[c], bx = r - by * sg.ch[c];
+  const int my = m / sg.mcux, mx = m - my * sg.mcux;
+  const int64_t bidx = (int64_t)(my * sg.cv[c] + by) * sg.cbw[c] + (mx * sg.ch[c] + bx);
+  return sg.coef + (sg.cb0[c] + bidx) * 64;
+}
+
+// Decode symbols from state st until the first codeword boundary at or past
+// end_bit (or until the tile's last block is complete).  g: index of the
+// block in progress (decode order), nblk: blocks completed.  With WRITE the
+// values are stored (DC as the difference, AC at its natural position).
+template <bool WRITE>
+__device__ JState jseg_run(const JSeg& sg, JState st, int64_t end_bit, int& g, int& nblk) {
+  LdsBits br{sg.s, sg.n, 0, 0, 0ull};
+  br.pos = (int)((st.bit >> 5) << 2);
+  br.refill();
+  br.refill();
+  br.skip((int)(st.bit & 31));
+  int u = st.u, z = st.z;
+  auto tell = [&]() { return (int64_t)br.pos * 8 - br.bits; };
+  int16_t* blk = nullptr;
+  if (WRITE && g < sg.total) blk = jseg_block(sg, g, u);
+  while (tell() < end_bit && g < sg.total) {
+    const int c = sg.ucomp[u];
+    if (z == 0) {
+      int sdc = huff_lds(br, sg.tabs[sg.dtab[c]]);
+      if (sdc > 15) sdc = 0;
+      const int diff = extend(br.get(sdc), sdc);
+      if (WRITE) blk[0] = (int16_t)diff;
+      z = 1;
+      continue;
+    }
+    const int rs = huff_lds(br, sg.tabs[sg.atab[c]]);
+    const int r = rs >> 4, sz = rs & 15;
+    bool done;
+    if (sz == 0) {
+      z += 16;
+      done = r != 15 || z >= 64;
+    } else {
+      z += r;
+      const int v = extend(br.get(sz), sz);
+      if (WRITE) blk[sg.zz[z < 63 ? z : 63]] = (int16_t)v;
+      ++z;
+      done = z >= 64;
+    }
+    if (done) {
+      z = 0;
+      u = u + 1 == sg.U ? 0 : u + 1;
+      ++g;
+      ++nblk;
+      if (WRITE && g < sg.total) blk = jseg_block(sg, g, u);
+    }
+  }
+  return JState{tell(), u, z};
+}
+
 __global__ __launch_bounds__(64) void k_jpeg_entropy_lds(const JDev a, int cap) {
   extern __shared__ __align__(16) uint8_t smem[];
   const int t = blockIdx.x;
@@ -890,7 +979,7 @@ __global__ __launch_bounds__(64) void k_jpeg_entropy_lds(const JDev a, int cap) 
   const JTile& jt = a.tiles[t];
   int16_t* coef = a.coef + jt.coef0 * 64;
   const int64_t n = jt.data_len;
-  if (jt.restart != 0 || n > cap) {
+  if (jt.restart != 0 || n > cap || n < 256) {
     if (lane == 0) jpeg_entropy(jt, a.data, coef);
     return;
   }
@@ -937,63 +1026,108 @@ __global__ __launch_bounds__(64) void k_jpeg_entropy_lds(const JDev a, int cap) 
     out += __shfl(incl, 63, 64);
   }
   __shared__ uint8_t zz[64];
+  __shared__ int8_t ucomp[12], ubx[12], uby[12];
   zz[lane] = (uint8_t)kZigzag[lane];
   if (lane < 8) ds[out + lane] = 0;  // the refill's second word past the end
+  const int ncomp = jt.ncomp;
+  int U = 0;
+  for (int c = 0; c < ncomp; ++c) U += jt.comp[c].h * jt.comp[c].v;
+  if (lane == 0) {  // units of an MCU: components in frame order, h x v blocks each
+    int u = 0;
+    for (int c = 0; c < ncomp; ++c)
+      for (int by = 0; by < jt.comp[c].v; ++by)
+        for (int bx = 0; bx < jt.comp[c].h; ++bx, ++u) {
+          ucomp[u] = (int8_t)c;
+          ubx[u] = (int8_t)bx;
+          uby[u] = (int8_t)by;
+        }
+  }
   __syncthreads();
-  if (lane != 0) return;
-  LdsBits br{ds, out, 0, 0, 0ull};
-  const int ncomp = jt.ncomp, mcux = jt.mcux;
-  const int64_t nmcu = (int64_t)mcux * jt.mcuy;
-  // per-component values in registers: the component loop is unrolled
-  int ch[3], cv[3], cbw[3], pred[3];
-  int64_t cb0[3];
-  const HuffTab* dtab[3];
-  const HuffTab* atab[3];
-#pragma unroll
+  JSeg seg;
+  seg.s = ds;
+  seg.n = out;
+  seg.tabs = tabs;
+  seg.zz = zz;
+  seg.ucomp = ucomp;
+  seg.ubx = ubx;
+  seg.uby = uby;
+  seg.U = U;
+  seg.mcux = jt.mcux;
+  seg.total = jt.mcux * jt.mcuy * U;
+  seg.coef = coef;
   for (int c = 0; c < 3; ++c) {
     const int cc = c < ncomp ? c : 0;
-    ch[c] = jt.comp[cc].h;
-    cv[c] = jt.comp[cc].v;
-    cbw[c] = jt.comp[cc].bw;
-    cb0[c] = jt.comp[cc].block0;
-    dtab[c] = tabs + jt.comp[cc].td;
-    atab[c] = tabs + 2 + jt.comp[cc].ta;
-    pred[c] = 0;
+    seg.dtab[c] = jt.comp[cc].td;
+    seg.atab[c] = 2 + jt.comp[cc].ta;
+    seg.ch[c] = jt.comp[cc].h;
+    seg.cv[c] = jt.comp[cc].v;
+    seg.cbw[c] = jt.comp[cc].bw;
+    seg.cb0[c] = jt.comp[cc].block0;
   }
-  int my = 0, mx = 0;
-  for (int64_t m = 0; m < nmcu; ++m) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      if (c < ncomp) {
-        const HuffTab& dct = *dtab[c];
-        const HuffTab& act = *atab[c];
-        for (int by = 0; by < cv[c]; ++by)
-          for (int bx = 0; bx < ch[c]; ++bx) {
-            const int64_t bidx = (int64_t)(my * cv[c] + by) * cbw[c] + (mx * ch[c] + bx);
-            int16_t* blk = coef + (cb0[c] + bidx) * 64;
-            int sdc = huff_lds(br, dct);
-            if (sdc > 15) sdc = 0;
-            pred[c] += extend(br.get(sdc), sdc);
-            blk[0] = (int16_t)pred[c];
-            for (int i = 1; i < 64;) {
-              const int rs = huff_lds(br, act);
-              const int r = rs >> 4, sz = rs & 15;
-              if (sz == 0) {
-                if (r != 15) break;
-                i += 16;
-                continue;
-              }
-              i += r;
-              if (i > 63) break;
-              blk[zz[i]] = (int16_t)extend(br.get(sz), sz);
-              ++i;
-            }
-          }
-      }
+  // each lane owns S bits (a multiple of 32) of the destuffed segment
+  const int64_t nbits = (int64_t)out * 8;
+  const int64_t S = ((nbits + 63) / 64 + 31) / 32 * 32;
+  const int64_t b0 = std::min<int64_t>((int64_t)lane * S, nbits);
+  const int64_t b1 = std::min<int64_t>(b0 + S, nbits);
+  // A: speculative decode of the own part from an assumed MCU start
+  JState st{b0, 0, 0};
+  int gb = 0;
+  int nblk = 0;
+  JState en = jseg_run<false>(seg, st, b1, gb, nblk);
+  // B: take the left neighbour's end state as the own start until no end
+  // state changes (lane 0's start is exact, so lane k is exact after k
+  // rounds; decoding self-synchronises, so a few rounds suffice)
+  for (int round = 0; round < 64; ++round) {
+    JState ps;
+    ps.bit = __shfl_up(en.bit, 1, 64);
+    ps.u = __shfl_up(en.u, 1, 64);
+    ps.z = __shfl_up(en.z, 1, 64);
+    if (lane == 0) ps = JState{0, 0, 0};
+    bool changed = false;
+    if (ps.bit != st.bit || ps.u != st.u || ps.z != st.z) {
+      st = ps;
+      gb = 0;
+      nblk = 0;
+      const JState e2 = jseg_run<false>(seg, st, b1, gb, nblk);
+      changed = e2.bit != en.bit || e2.u != en.u || e2.z != en.z;
+      en = e2;
     }
-    if (++mx == mcux) {
-      mx = 0;
-      ++my;
+    if (!__any(changed)) break;
+  }
+  // C: first block of each part = blocks completed by the parts before it
+  int incl = nblk;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += v;
+  }
+  gb = incl - nblk;
+  // D: decode again, storing coefficients (DC as differences)
+  nblk = 0;
+  jseg_run<true>(seg, st, b1, gb, nblk);
+  __threadfence_block();
+  __syncthreads();
+  // E: DC prediction = running sum of the differences per component, in
+  // decode order
+  const int nmcu = jt.mcux * jt.mcuy;
+  for (int c = 0; c < ncomp; ++c) {
+    const int hv = seg.ch[c] * seg.cv[c];
+    const int N = nmcu * hv;
+    const int per = (N + 63) / 64;
+    const int k0 = min(lane * per, N), k1 = min(k0 + per, N);
+    int sum = 0;
+    for (int k = k0; k < k1; ++k) sum += *jseg_dc(seg, c, hv, k);
+    int inc = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += v;
+    }
+    int run = inc - sum;
+    for (int k = k0; k < k1; ++k) {
+      int16_t* p = jseg_dc(seg, c, hv, k);
+      run += *p;
+      *p = (int16_t)run;
     }
   }
 }
