@@ -676,7 +676,9 @@ def bench_wsi(reps=5, cpu=True):
             torch.cuda.synchronize()
             return (time.perf_counter() - t0) / n
 
-        rd = lambda: im.read_region((0, 0), (edge, edge), 0, device="cuda", buf=out)  # noqa: E731
+        nw = min(16, os.cpu_count() or 1)
+        rd = lambda: im.read_region((0, 0), (edge, edge), 0, device="cuda", buf=out,  # noqa: E731
+                                    num_workers=nw)
         t = timed(rd, reps)
         res["device_read_ms"] = t * 1e3
         res["device_read_mpx_s"] = edge * edge / t / 1e6
@@ -687,7 +689,7 @@ def bench_wsi(reps=5, cpu=True):
         pipe = lambda: filters.gaussian(color.rgb2gray(rd()), sigma=2.0)  # noqa: E731
         t = timed(pipe, reps)
         res["read_gray_gaussian_ms"] = t * 1e3
-        nw = min(16, os.cpu_count() or 1)
+        res["host_threads_for_tile_reads"] = nw
         hb = np.empty((edge, edge, 3), np.uint8)
         im.read_region((0, 0), (edge, edge), 0, num_workers=nw, buf=hb)
         t0 = time.perf_counter()

@@ -24,6 +24,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -354,10 +355,16 @@ struct JTile {
   int transform;               // 1: YCbCr -> RGB, 0: components as stored
   int64_t nblocks;
   int64_t coef0;               // first DCT block in the coefficient buffer
-  int32_t qtab[4][64];         // natural order
-  HuffTab dc[2], ac[2];
+  int tabset;                  // index of the tile's JTables
   int x0, y0;                  // tile origin in the level
   int vis_w, vis_h;            // pixels of the tile inside the level
+};
+
+// Quantization (natural order) and Huffman tables in force for a tile: a
+// level's JPEGTables stream, overridden by tables inside the tile's stream.
+struct JTables {
+  int32_t qtab[4][64];
+  HuffTab dc[2], ac[2];
 };
 
 bool build_huff(const uint8_t* counts, const uint8_t* vals, int nvals, HuffTab& t) {
@@ -391,16 +398,19 @@ bool build_huff(const uint8_t* counts, const uint8_t* vals, int nvals, HuffTab& 
   return true;
 }
 
-// Marker-level parse of the tables stream and one tile's stream.  Fills the
-// frame, scan and tables of jt; sets the entropy data offset / length
-// relative to the tile stream.
-int parse_jpeg(const uint8_t* tables, size_t tlen, const uint8_t* s, size_t n, JTile& jt) {
-  std::memset(&jt, 0, sizeof(jt));
-  jt.transform = -1;
+// Marker-level JPEG parser.  segs() reads the segments of one stream: DQT /
+// DHT into tb, the frame and scan into jt (a tables-only stream has neither);
+// finish() checks the tables the scan references and derives the MCU
+// geometry.  A level's JPEGTables stream is parsed once; each tile's stream
+// is parsed on top of a copy of the resulting state.
+struct JParser {
+  JTables* tb = nullptr;
   bool have_q[4] = {false, false, false, false};
   bool have_dc[2] = {false, false}, have_ac[2] = {false, false};
-  bool have_frame = false, saw_jfif = false;
-  auto seg = [&](const uint8_t* b, size_t len, bool tile) -> int {
+  bool have_frame = false, saw_jfif = false, own_tables = false;
+  int adobe = -1;
+
+  int segs(const uint8_t* b, size_t len, bool tile, JTile& jt) {
     size_t p = 0;
     while (p + 1 < len) {
       if (b[p] != 0xFF) {
@@ -417,10 +427,12 @@ int parse_jpeg(const uint8_t* tables, size_t tlen, const uint8_t* s, size_t n, J
       const uint8_t* d = b + p + 2;
       const size_t dl = L - 2;
       if (m == 0xDB) {  // DQT
+        if (tile) own_tables = true;
         size_t q = 0;
         while (q < dl) {
           const int pq = d[q] >> 4, tq = d[q] & 15;
           if (tq > 3) return fail(HC_ERR_VALUE, "JPEG quantization table id");
+          if (q + 1 + (pq ? 128 : 64) > dl) return fail(HC_ERR_VALUE, "truncated JPEG DQT");
           ++q;
           for (int i = 0; i < 64; ++i) {
             int v;
@@ -430,11 +442,12 @@ int parse_jpeg(const uint8_t* tables, size_t tlen, const uint8_t* s, size_t n, J
             } else {
               v = d[q++];
             }
-            jt.qtab[tq][kZigzag[i]] = v;
+            tb->qtab[tq][kZigzag[i]] = v;
           }
           have_q[tq] = true;
         }
       } else if (m == 0xC4) {  // DHT
+        if (tile) own_tables = true;
         size_t q = 0;
         while (q + 17 <= dl) {
           const int tc = d[q] >> 4, th = d[q] & 15;
@@ -442,19 +455,21 @@ int parse_jpeg(const uint8_t* tables, size_t tlen, const uint8_t* s, size_t n, J
           int nv = 0;
           for (int i = 0; i < 16; ++i) nv += d[q + 1 + i];
           if (q + 17 + (size_t)nv > dl) return fail(HC_ERR_VALUE, "truncated JPEG Huffman table");
-          HuffTab& t = tc ? jt.ac[th] : jt.dc[th];
+          HuffTab& t = tc ? tb->ac[th] : tb->dc[th];
           if (!build_huff(d + q + 1, d + q + 17, nv, t))
             return fail(HC_ERR_VALUE, "bad JPEG Huffman table");
           (tc ? have_ac : have_dc)[th] = true;
           q += 17 + (size_t)nv;
         }
       } else if (m == 0xC0 || m == 0xC1) {  // baseline / extended sequential
+        if (dl < 6) return fail(HC_ERR_VALUE, "truncated JPEG frame");
         if (d[0] != 8) return fail(HC_ERR_UNSUPPORTED, "JPEG sample precision other than 8");
         jt.height = (d[1] << 8) | d[2];
         jt.width = (d[3] << 8) | d[4];
         jt.ncomp = d[5];
         if (jt.ncomp != 1 && jt.ncomp != 3)
           return fail(HC_ERR_UNSUPPORTED, "JPEG with other than 1 or 3 components");
+        if (dl < 6 + 3 * (size_t)jt.ncomp) return fail(HC_ERR_VALUE, "truncated JPEG frame");
         for (int c = 0; c < jt.ncomp; ++c) {
           jt.comp[c].id = d[6 + 3 * c];
           jt.comp[c].h = d[7 + 3 * c] >> 4;
@@ -467,15 +482,17 @@ int parse_jpeg(const uint8_t* tables, size_t tlen, const uint8_t* s, size_t n, J
       } else if (m >= 0xC2 && m <= 0xCF && m != 0xC4 && m != 0xC8 && m != 0xCC) {
         return fail(HC_ERR_UNSUPPORTED, "progressive, lossless or arithmetic JPEG");
       } else if (m == 0xDD) {  // DRI
+        if (dl < 2) return fail(HC_ERR_VALUE, "truncated JPEG DRI");
         jt.restart = (d[0] << 8) | d[1];
       } else if (m == 0xE0 && dl >= 5 && std::memcmp(d, "JFIF", 5) == 0) {
         saw_jfif = true;
       } else if (m == 0xEE && dl >= 12 && std::memcmp(d, "Adobe", 5) == 0) {
-        jt.transform = d[11] == 0 ? 0 : 1;
+        adobe = d[11] == 0 ? 0 : 1;
       } else if (m == 0xDA) {  // SOS
         if (!tile || !have_frame) return fail(HC_ERR_VALUE, "JPEG scan before the frame");
         const int ns = d[0];
         if (ns != jt.ncomp) return fail(HC_ERR_UNSUPPORTED, "JPEG with more than one scan");
+        if (dl < 1 + 2 * (size_t)ns) return fail(HC_ERR_VALUE, "truncated JPEG scan header");
         for (int i = 0; i < ns; ++i) {
           const int cs = d[1 + 2 * i], t = d[2 + 2 * i];
           int c = 0;
@@ -490,54 +507,85 @@ int parse_jpeg(const uint8_t* tables, size_t tlen, const uint8_t* s, size_t n, J
         // entropy data runs to the EOI marker (restart markers inside)
         size_t e = p + L;
         while (e + 1 < len && !(b[e] == 0xFF && b[e + 1] == 0xD9)) ++e;
+        if (e + 1 >= len) e = len;  // no EOI: the data runs to the end
         jt.data_len = (int64_t)(e - (p + L));
         return HC_OK;
       }
       p += L;
     }
     return tile ? fail(HC_ERR_VALUE, "JPEG tile without a scan") : HC_OK;
-  };
+  }
+
+  int finish(JTile& jt) {
+    for (int c = 0; c < jt.ncomp; ++c) {
+      if (!have_q[jt.comp[c].tq] || !have_dc[jt.comp[c].td] || !have_ac[jt.comp[c].ta])
+        return fail(HC_ERR_VALUE, "JPEG table referenced but not defined");
+    }
+    // a one-component scan is non-interleaved: one block per MCU in raster
+    // order whatever the frame's sampling factors (JPEG spec A.2.2)
+    if (jt.ncomp == 1) jt.comp[0].h = jt.comp[0].v = 1;
+    jt.hmax = jt.vmax = 1;
+    for (int c = 0; c < jt.ncomp; ++c) {
+      jt.hmax = std::max(jt.hmax, jt.comp[c].h);
+      jt.vmax = std::max(jt.vmax, jt.comp[c].v);
+    }
+    jt.mcux = (jt.width + 8 * jt.hmax - 1) / (8 * jt.hmax);
+    jt.mcuy = (jt.height + 8 * jt.vmax - 1) / (8 * jt.vmax);
+    int64_t nb = 0;
+    for (int c = 0; c < jt.ncomp; ++c) {
+      JComp& k = jt.comp[c];
+      k.bw = jt.mcux * k.h;
+      k.bh = jt.mcuy * k.v;
+      k.block0 = nb;
+      nb += (int64_t)k.bw * k.bh;
+    }
+    jt.nblocks = nb;
+    // colour space of a three-component stream as libjpeg's header defaults
+    // decide: JFIF -> YCbCr; else an Adobe marker's transform (0: RGB);
+    // else component ids 'R','G','B' -> RGB; else YCbCr
+    if (jt.ncomp != 3) {
+      jt.transform = 0;
+    } else if (saw_jfif) {
+      jt.transform = 1;
+    } else if (adobe >= 0) {
+      jt.transform = adobe;
+    } else {
+      const bool rgb_ids = jt.comp[0].id == 'R' && jt.comp[1].id == 'G' && jt.comp[2].id == 'B';
+      jt.transform = rgb_ids ? 0 : 1;
+    }
+    return HC_OK;
+  }
+};
+
+// Whether a tile stream carries its own DQT / DHT segments before its scan.
+bool stream_has_tables(const uint8_t* b, size_t len) {
+  size_t p = 2;
+  while (p + 4 <= len && b[p] == 0xFF) {
+    const int m = b[p + 1];
+    if (m == 0xDA) return false;
+    if (m == 0xDB || m == 0xC4) return true;
+    if (m == 0xFF) {
+      ++p;
+      continue;
+    }
+    p += 2 + (((size_t)b[p + 2] << 8) | b[p + 3]);
+  }
+  return true;  // unusual layout: parse it fully
+}
+
+// Whole parse of one tile: the tables stream, then the tile's stream.
+int parse_jpeg(const uint8_t* tables, size_t tlen, const uint8_t* s, size_t n, JTile& jt,
+               JTables& tb) {
+  std::memset(&jt, 0, sizeof(jt));
+  JParser ps;
+  ps.tb = &tb;
   if (tables && tlen) {
-    const int rc = seg(tables, tlen, false);
+    const int rc = ps.segs(tables, tlen, false, jt);
     if (rc != HC_OK) return rc;
   }
-  const int rc = seg(s, n, true);
+  const int rc = ps.segs(s, n, true, jt);
   if (rc != HC_OK) return rc;
-  for (int c = 0; c < jt.ncomp; ++c) {
-    if (!have_q[jt.comp[c].tq] || !have_dc[jt.comp[c].td] || !have_ac[jt.comp[c].ta])
-      return fail(HC_ERR_VALUE, "JPEG table referenced but not defined");
-  }
-  // a one-component scan is non-interleaved: one block per MCU in raster
-  // order whatever the frame's sampling factors (JPEG spec A.2.2)
-  if (jt.ncomp == 1) jt.comp[0].h = jt.comp[0].v = 1;
-  jt.hmax = jt.vmax = 1;
-  for (int c = 0; c < jt.ncomp; ++c) {
-    jt.hmax = std::max(jt.hmax, jt.comp[c].h);
-    jt.vmax = std::max(jt.vmax, jt.comp[c].v);
-  }
-  jt.mcux = (jt.width + 8 * jt.hmax - 1) / (8 * jt.hmax);
-  jt.mcuy = (jt.height + 8 * jt.vmax - 1) / (8 * jt.vmax);
-  int64_t nb = 0;
-  for (int c = 0; c < jt.ncomp; ++c) {
-    JComp& k = jt.comp[c];
-    k.bw = jt.mcux * k.h;
-    k.bh = jt.mcuy * k.v;
-    k.block0 = nb;
-    nb += (int64_t)k.bw * k.bh;
-  }
-  jt.nblocks = nb;
-  // colour space of a three-component stream as libjpeg's header defaults
-  // decide: JFIF -> YCbCr; else an Adobe marker's transform (0: RGB);
-  // else component ids 'R','G','B' -> RGB; else YCbCr
-  if (jt.ncomp != 3) {
-    jt.transform = 0;
-  } else if (saw_jfif) {
-    jt.transform = 1;
-  } else if (jt.transform < 0) {
-    const bool rgb_ids = jt.comp[0].id == 'R' && jt.comp[1].id == 'G' && jt.comp[2].id == 'B';
-    jt.transform = rgb_ids ? 0 : 1;
-  }
-  return HC_OK;
+  return ps.finish(jt);
 }
 
 // Bit reader over an entropy-coded segment: 0xFF 0x00 is a data 0xFF;
@@ -614,7 +662,8 @@ __host__ __device__ inline int extend(int v, int s) {
 
 // Entropy-decode one tile into quantized coefficients (natural order),
 // block b of component c at coef[(block0_c + b) * 64].
-__host__ __device__ void jpeg_entropy(const JTile& jt, const uint8_t* data, int16_t* coef) {
+__host__ __device__ void jpeg_entropy(const JTile& jt, const JTables& tb, const uint8_t* data,
+                                    int16_t* coef) {
   BitReader br{data + jt.data_off, jt.data_len, 0, 0u, 0};
   int pred[3] = {0, 0, 0};
   const int64_t nmcu = (int64_t)jt.mcux * jt.mcuy;
@@ -628,8 +677,8 @@ __host__ __device__ void jpeg_entropy(const JTile& jt, const uint8_t* data, int1
     const int my = (int)(m / jt.mcux), mx = (int)(m - (int64_t)my * jt.mcux);
     for (int c = 0; c < jt.ncomp; ++c) {
       const JComp& k = jt.comp[c];
-      const HuffTab& dct = jt.dc[k.td];
-      const HuffTab& act = jt.ac[k.ta];
+      const HuffTab& dct = tb.dc[k.td];
+      const HuffTab& act = tb.ac[k.ta];
       // non-interleaved single-component scans hold blocks in raster order;
       // one component with h = v = 1 is the same as interleaved order
       for (int by = 0; by < k.v; ++by)
@@ -808,10 +857,10 @@ __host__ __device__ void jpeg_pixel(const JTile& jt, uint8_t* const* planes, int
 // ------------------------------------------------------- device side --
 struct JDev {
   const JTile* tiles;
+  const JTables* tabsets;
   const uint8_t* data;
   int16_t* coef;
-  uint8_t* planes;      // decoded component planes, per tile at plane0
-  const int64_t* plane0;
+  uint8_t* planes;      // decoded component planes: a tile's at coef0 * 64
   int ntiles;
   uint8_t* out;         // region, rows of rw * nch bytes
   int64_t rx, ry, rw, rh;
@@ -980,14 +1029,15 @@ __global__ __launch_bounds__(64) void k_jpeg_entropy_lds(const JDev a, int cap) 
   int16_t* coef = a.coef + jt.coef0 * 64;
   const int64_t n = jt.data_len;
   if (jt.restart != 0 || n > cap || n < 256) {
-    if (lane == 0) jpeg_entropy(jt, a.data, coef);
+    if (lane == 0) jpeg_entropy(jt, a.tabsets[jt.tabset], a.data, coef);
     return;
   }
   // tables: dc[0], dc[1], ac[0], ac[1] as 32-bit words
   HuffTab* tabs = reinterpret_cast<HuffTab*>(smem);
   {
-    const uint32_t* src0 = reinterpret_cast<const uint32_t*>(&jt.dc[0]);
-    const uint32_t* src1 = reinterpret_cast<const uint32_t*>(&jt.ac[0]);
+    const JTables& tb = a.tabsets[jt.tabset];
+    const uint32_t* src0 = reinterpret_cast<const uint32_t*>(&tb.dc[0]);
+    const uint32_t* src1 = reinterpret_cast<const uint32_t*>(&tb.ac[0]);
     uint32_t* dst = reinterpret_cast<uint32_t*>(smem);
     constexpr int half = 2 * (int)sizeof(HuffTab) / 4;
     for (int i = lane; i < half; i += 64) {
@@ -1139,25 +1189,31 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, int64_t n, uint32
 
 // One work-group per 8x8 block of a tile: the inverse transform into the
 // tile's component plane.
-__global__ __launch_bounds__(64) void k_jpeg_idct(const JDev a, const int64_t* blk_tile,
-                                                  int64_t nblk) {
+__global__ __launch_bounds__(64) void k_jpeg_idct(const JDev a, int64_t nblk) {
   for (int64_t g = blockIdx.x * 64 + threadIdx.x; g < nblk; g += (int64_t)gridDim.x * 64) {
-    const int t = (int)blk_tile[g];
-    const JTile& jt = a.tiles[t];
+    // the tile holding block g: the last one whose first block is <= g
+    int lo = 0, hi = a.ntiles - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.tiles[mid].coef0 <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    const JTile& jt = a.tiles[lo];
     const int64_t b = g - jt.coef0;  // block within the tile
     int c = 0;
     while (c + 1 < jt.ncomp && b >= jt.comp[c + 1].block0) ++c;
     const JComp& k = jt.comp[c];
     const int64_t lb = b - k.block0;
     const int by = (int)(lb / k.bw), bx = (int)(lb - (int64_t)by * k.bw);
-    uint8_t* plane = a.planes + a.plane0[t] + k.block0 * 64;
+    uint8_t* plane = a.planes + (jt.coef0 + k.block0) * 64;
     const int pw = k.bw * 8;
-    idct_islow(a.coef + g * 64, jt.qtab[k.tq], plane + (int64_t)by * 8 * pw + bx * 8, pw);
+    idct_islow(a.coef + g * 64, a.tabsets[jt.tabset].qtab[k.tq],
+               plane + (int64_t)by * 8 * pw + bx * 8, pw);
   }
 }
 
 // One thread per region pixel covered by a JPEG tile.
-__global__ __launch_bounds__(256) void k_jpeg_color(const JDev a, const int64_t* pix0) {
+__global__ __launch_bounds__(256) void k_jpeg_color(const JDev a) {
   const int t = blockIdx.y;
   const JTile& jt = a.tiles[t];
   // the tile's overlap with the region
@@ -1167,7 +1223,7 @@ __global__ __launch_bounds__(256) void k_jpeg_color(const JDev a, const int64_t*
   const int64_t ow = ox1 - ox0, oh = oy1 - oy0;
   if (ow <= 0 || oh <= 0) return;
   uint8_t* planes[3];
-  for (int c = 0; c < jt.ncomp; ++c) planes[c] = a.planes + a.plane0[t] + jt.comp[c].block0 * 64;
+  for (int c = 0; c < jt.ncomp; ++c) planes[c] = a.planes + (jt.coef0 + jt.comp[c].block0) * 64;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < ow * oh;
        i += (int64_t)gridDim.x * 256) {
     const int64_t yy = i / ow, xx = i - yy * ow;
@@ -1175,7 +1231,6 @@ __global__ __launch_bounds__(256) void k_jpeg_color(const JDev a, const int64_t*
     jpeg_pixel(jt, planes, (int)(gx - jt.x0), (int)(gy - jt.y0),
                a.out + ((gy - a.ry) * a.rw + (gx - a.rx)) * a.nch, a.nch);
   }
-  (void)pix0;
 }
 
 __global__ __launch_bounds__(256) void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
@@ -1259,19 +1314,20 @@ void set_color(JTile& jt, const Ifd& f) {
 int decode_jpeg_host(const Ifd& f, const std::vector<uint8_t>& src, int nch,
                      std::vector<uint8_t>& px) {
   JTile jt;
-  int rc = parse_jpeg(f.jpeg_tables.data(), f.jpeg_tables.size(), src.data(), src.size(), jt);
+  JTables tb;
+  int rc = parse_jpeg(f.jpeg_tables.data(), f.jpeg_tables.size(), src.data(), src.size(), jt, tb);
   if (rc != HC_OK) return rc;
   set_color(jt, f);
   if (jt.ncomp > nch) return fail(HC_ERR_VALUE, "JPEG tile has more components than the TIFF samples");
   std::vector<int16_t> coef((size_t)jt.nblocks * 64);
-  jpeg_entropy(jt, src.data(), coef.data());
+  jpeg_entropy(jt, tb, src.data(), coef.data());
   std::vector<uint8_t> planes((size_t)jt.nblocks * 64);
   for (int c = 0; c < jt.ncomp; ++c) {
     const JComp& k = jt.comp[c];
     uint8_t* plane = planes.data() + k.block0 * 64;
     for (int64_t b = 0; b < (int64_t)k.bw * k.bh; ++b) {
       const int by = (int)(b / k.bw), bx = (int)(b % k.bw);
-      idct_islow(coef.data() + (k.block0 + b) * 64, jt.qtab[k.tq],
+      idct_islow(coef.data() + (k.block0 + b) * 64, tb.qtab[k.tq],
                  plane + (int64_t)by * 8 * k.bw * 8 + bx * 8, k.bw * 8);
     }
   }
@@ -1335,6 +1391,18 @@ using namespace hc;
 
 struct hc_wsi {
   Wsi w;
+  // staging (pinned host) and work (device) buffers of device reads, kept
+  // between calls and grown as needed; device reads on one handle serialise
+  mutable std::mutex mu;
+  mutable uint8_t* pinned = nullptr;
+  mutable size_t pinned_cap = 0;
+  mutable uint8_t* dev = nullptr;
+  mutable size_t dev_cap = 0;
+  mutable int dev_id = -1;
+  ~hc_wsi() {
+    if (pinned) (void)hipHostFree(pinned);
+    if (dev) (void)hipFree(dev);
+  }
 };
 
 extern "C" {
@@ -1414,47 +1482,148 @@ int hc_wsi_read_region_device(const hc_wsi* h, int level, int64_t x, int64_t y, 
   hipStream_t st = (hipStream_t)stream;
   const int64_t nbytes = w * hgt * nch;
   if (f.compression != 7) {
-    // lossless tiles: decoded on host threads, one upload of the region
-    uint8_t* host = nullptr;
-    HC_HIP(hipHostMalloc((void**)&host, (size_t)nbytes));
-    rc = read_region_host(ws, level, x, y, w, hgt, host, num_workers);
+    // lossless tiles: decoded on host threads into the handle's pinned
+    // staging buffer, one upload of the region
+    std::lock_guard<std::mutex> lock(h->mu);
+    if (h->pinned_cap < (size_t)nbytes) {
+      if (h->pinned) (void)hipHostFree(h->pinned);
+      h->pinned = nullptr;
+      h->pinned_cap = 0;
+      HC_HIP(hipHostMalloc((void**)&h->pinned, (size_t)nbytes));
+      h->pinned_cap = (size_t)nbytes;
+    }
+    rc = read_region_host(ws, level, x, y, w, hgt, h->pinned, num_workers);
     if (rc == HC_OK) {
-      hipError_t e = hipMemcpyAsync(out_dev, host, (size_t)nbytes, hipMemcpyHostToDevice, st);
+      hipError_t e = hipMemcpyAsync(out_dev, h->pinned, (size_t)nbytes, hipMemcpyHostToDevice, st);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       if (e != hipSuccess) rc = hip_check(e, "region upload");
     }
-    (void)hipHostFree(host);
     return rc;
   }
-  // JPEG tiles: headers parsed here, entropy decoding and transforms on the GPU
+  // JPEG tiles: byte ranges read and headers parsed on host threads, entropy
+  // decoding and transforms on the GPU
   std::vector<TileRef> tiles;
   tiles_of_region(f, x, y, w, hgt, tiles);
   if (tiles.size() > 65535) return fail(HC_ERR_RESOURCE, "region spans more than 65535 JPEG tiles");
   const int nt = (int)tiles.size();
-  std::vector<JTile> jt(nt);
-  std::vector<uint8_t> blob;
-  std::vector<int64_t> plane0(nt);
-  int64_t nblk = 0, nplane = 0;
+  std::lock_guard<std::mutex> lock(h->mu);
+  // the level's shared tables, parsed once
+  JTables base;
+  std::memset(&base, 0, sizeof(base));
+  JParser bp;
+  bp.tb = &base;
   {
-    std::vector<uint8_t> src;
-    for (int i = 0; i < nt; ++i) {
-      if (!read_tile_bytes(ws, f, tiles[i], src)) return fail(HC_ERR_VALUE, "tile read failed");
-      rc = parse_jpeg(f.jpeg_tables.data(), f.jpeg_tables.size(), src.data(), src.size(), jt[i]);
+    JTile dummy;
+    std::memset(&dummy, 0, sizeof(dummy));
+    if (!f.jpeg_tables.empty()) {
+      rc = bp.segs(f.jpeg_tables.data(), f.jpeg_tables.size(), false, dummy);
       if (rc != HC_OK) return rc;
-      set_color(jt[i], f);
-      if (jt[i].ncomp > nch)
-        return fail(HC_ERR_VALUE, "JPEG tile has more components than the TIFF samples");
-      jt[i].data_off += (int64_t)blob.size();
-      blob.insert(blob.end(), src.begin(), src.end());
-      jt[i].coef0 = nblk;
-      plane0[i] = nplane;
-      nblk += jt[i].nblocks;
-      nplane += jt[i].nblocks * 64;
-      jt[i].x0 = (int)(tiles[i].tx * f.tile_w);
-      jt[i].y0 = (int)(tiles[i].ty * f.tile_h);
-      jt[i].vis_w = (int)std::min<int64_t>(jt[i].width, std::min(f.tile_w, f.width - jt[i].x0));
-      jt[i].vis_h = (int)std::min<int64_t>(jt[i].height, std::min(f.tile_h, f.height - jt[i].y0));
     }
+  }
+  // staging layout: tiles | table sets (at most one per tile + the shared
+  // one) | compressed tiles back to back
+  auto al = [](int64_t v) { return (v + 255) / 256 * 256; };
+  std::vector<int64_t> boff(nt + 1, 0), foff(nt);
+  for (int i = 0; i < nt; ++i) {
+    const int64_t idx = tiles[i].ty * f.tiles_x() + tiles[i].tx;
+    if (f.counts[idx] > (1ull << 31)) return fail(HC_ERR_VALUE, "implausible tile byte count");
+    foff[i] = (int64_t)f.offsets[idx];
+    boff[i + 1] = boff[i] + (int64_t)f.counts[idx];
+  }
+  const int64_t o_tabs = al((int64_t)sizeof(JTile) * std::max(nt, 1));
+  const int64_t o_blob = al(o_tabs + (int64_t)sizeof(JTables) * (nt + 1));
+  const size_t staging = (size_t)(o_blob + boff[nt] + 16);
+  if (h->pinned_cap < staging) {
+    if (h->pinned) (void)hipHostFree(h->pinned);
+    h->pinned = nullptr;
+    h->pinned_cap = 0;
+    const size_t cap = staging + staging / 4;
+    HC_HIP(hipHostMalloc((void**)&h->pinned, cap));
+    h->pinned_cap = cap;
+  }
+  uint8_t* hp = h->pinned;
+  JTile* T = reinterpret_cast<JTile*>(hp);
+  JTables* TB = reinterpret_cast<JTables*>(hp + o_tabs);
+  uint8_t* blob = hp + o_blob;
+  TB[0] = base;
+  // 1) reads: runs of tiles that lie back to back in the file, one pread each
+  std::vector<std::pair<int, int>> runs;  // [first, last] tile of a run
+  for (int i = 0; i < nt; ++i) {
+    if (!runs.empty() && foff[i] == foff[runs.back().second] + (boff[runs.back().second + 1] -
+                                                                 boff[runs.back().second]))
+      runs.back().second = i;
+    else
+      runs.push_back({i, i});
+  }
+  const int workers = std::max(1, num_workers);
+  std::atomic<int> err{HC_OK};
+  std::mutex emu;
+  std::string emsg;
+  auto set_err = [&](int r) {
+    std::lock_guard<std::mutex> lk(emu);
+    if (err.load() == HC_OK) {
+      emsg = hc_last_error();
+      err.store(r);
+    }
+  };
+  auto parallel = [&](int n, const std::function<void(int)>& fn) {
+    std::atomic<int> next{0};
+    auto body = [&]() {
+      for (int i = next++; i < n; i = next++) {
+        if (err.load() != HC_OK) return;
+        fn(i);
+      }
+    };
+    const int nth = std::min(workers, n);
+    if (nth <= 1) {
+      body();
+      return;
+    }
+    std::vector<std::thread> th;
+    for (int k = 0; k < nth; ++k) th.emplace_back(body);
+    for (auto& t : th) t.join();
+  };
+  parallel((int)runs.size(), [&](int r) {
+    const int a0 = runs[r].first, a1 = runs[r].second;
+    if (!read_at(ws.fd, (uint64_t)foff[a0], blob + boff[a0], (size_t)(boff[a1 + 1] - boff[a0])))
+      set_err(fail(HC_ERR_VALUE, "tile read failed"));
+  });
+  // 2) headers: each tile's stream on top of the shared tables
+  std::atomic<int> nsets{1};
+  parallel(nt, [&](int i) {
+    JTile& j = T[i];
+    std::memset(&j, 0, sizeof(j));
+    JParser ps = bp;
+    JTables unused;
+    ps.tb = &unused;
+    const uint8_t* src = blob + boff[i];
+    const size_t n = (size_t)(boff[i + 1] - boff[i]);
+    if (stream_has_tables(src, n)) {
+      const int k = nsets++;
+      TB[k] = base;
+      ps.tb = &TB[k];
+      j.tabset = k;
+    }
+    int r = ps.segs(src, n, true, j);
+    if (r == HC_OK) r = ps.finish(j);
+    if (r == HC_OK && j.ncomp > nch)
+      r = fail(HC_ERR_VALUE, "JPEG tile has more components than the TIFF samples");
+    if (r != HC_OK) {
+      set_err(r);
+      return;
+    }
+    set_color(j, f);
+    j.data_off += boff[i];
+    j.x0 = (int)(tiles[i].tx * f.tile_w);
+    j.y0 = (int)(tiles[i].ty * f.tile_h);
+    j.vis_w = (int)std::min<int64_t>(j.width, std::min(f.tile_w, f.width - j.x0));
+    j.vis_h = (int)std::min<int64_t>(j.height, std::min(f.tile_h, f.height - j.y0));
+  });
+  if (err.load() != HC_OK) return fail(err.load(), emsg);
+  int64_t nblk = 0;
+  for (int i = 0; i < nt; ++i) {
+    T[i].coef0 = nblk;
+    nblk += T[i].nblocks;
   }
   // LDS staging capacity of the entropy kernel: the largest segment of a
   // tile without restart markers, rounded up, at most kMaxStage bytes (larger
@@ -1462,43 +1631,39 @@ int hc_wsi_read_region_device(const hc_wsi* h, int level, int64_t x, int64_t y, 
   constexpr int64_t kMaxStage = 120 * 1024;
   int64_t need = 0;
   for (int i = 0; i < nt; ++i)
-    if (jt[i].restart == 0 && jt[i].data_len <= kMaxStage) need = std::max(need, jt[i].data_len);
+    if (T[i].restart == 0 && T[i].data_len <= kMaxStage) need = std::max(need, T[i].data_len);
   const int cap = (int)((need + 16 * 64 - 1) / (16 * 64) * (16 * 64));
-  std::vector<int64_t> blk_tile((size_t)std::max<int64_t>(nblk, 1));
-  for (int i = 0; i < nt; ++i)
-    for (int64_t b = 0; b < jt[i].nblocks; ++b) blk_tile[jt[i].coef0 + b] = i;
-  // device buffers (one allocation) and the uploads
-  auto al = [](int64_t v) { return (v + 255) / 256 * 256; };
-  const int64_t o_tiles = 0, o_blob = al(o_tiles + (int64_t)sizeof(JTile) * std::max(nt, 1));
-  const int64_t o_p0 = al(o_blob + (int64_t)blob.size() + 1);
-  const int64_t o_bt = al(o_p0 + 8 * std::max(nt, 1));
-  const int64_t o_coef = al(o_bt + 8 * (int64_t)blk_tile.size());
+  // device work buffer: the staged bytes at the same offsets, then the
+  // coefficients and the component planes
+  const int64_t o_coef = al(o_blob + boff[nt] + 16);
   const int64_t o_pl = al(o_coef + 128 * std::max<int64_t>(nblk, 1));
-  const int64_t total = al(o_pl + std::max<int64_t>(nplane, 1));
-  uint8_t* dev = nullptr;
-  HC_HIP(hipMalloc((void**)&dev, (size_t)total));
-  auto cleanup = [&]() { (void)hipFree(dev); };
+  const size_t total = (size_t)al(o_pl + 64 * std::max<int64_t>(nblk, 1));
+  int dev_id = 0;
+  HC_HIP(hipGetDevice(&dev_id));
+  if (h->dev_cap < total || h->dev_id != dev_id) {
+    if (h->dev) (void)hipFree(h->dev);
+    h->dev = nullptr;
+    h->dev_cap = 0;
+    const size_t capb = total + total / 4;
+    HC_HIP(hipMalloc((void**)&h->dev, capb));
+    h->dev_cap = capb;
+    h->dev_id = dev_id;
+  }
+  uint8_t* dev = h->dev;
   hipError_t e = hipSuccess;
   if (nt > 0) {
-    e = hipMemcpyAsync(dev + o_tiles, jt.data(), sizeof(JTile) * nt, hipMemcpyHostToDevice, st);
+    e = hipMemcpyAsync(dev, hp, (size_t)(o_tabs + (int64_t)sizeof(JTables) * nsets.load()),
+                       hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(dev + o_blob, blob.data(), blob.size(), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(dev + o_p0, plane0.data(), 8 * nt, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(dev + o_bt, blk_tile.data(), 8 * blk_tile.size(), hipMemcpyHostToDevice,
-                         st);
+      e = hipMemcpyAsync(dev + o_blob, blob, (size_t)boff[nt], hipMemcpyHostToDevice, st);
   }
-  if (e != hipSuccess) {
-    cleanup();
-    return hip_check(e, "JPEG tile upload");
-  }
+  if (e != hipSuccess) return hip_check(e, "JPEG tile upload");
   JDev a;
-  a.tiles = reinterpret_cast<const JTile*>(dev + o_tiles);
+  a.tiles = reinterpret_cast<const JTile*>(dev);
+  a.tabsets = reinterpret_cast<const JTables*>(dev + o_tabs);
   a.data = dev + o_blob;
   a.coef = reinterpret_cast<int16_t*>(dev + o_coef);
   a.planes = dev + o_pl;
-  a.plane0 = reinterpret_cast<const int64_t*>(dev + o_p0);
   a.ntiles = nt;
   a.out = out_dev;
   a.rx = x;
@@ -1524,18 +1689,17 @@ int hc_wsi_read_region_device(const hc_wsi* h, int level, int64_t x, int64_t y, 
     HC_LAUNCH("k_jpeg_entropy_lds", st, k_jpeg_entropy_lds, dim3((unsigned)nt), dim3(64), smem,
               a, cap);
     HC_LAUNCH("k_jpeg_idct", st, k_jpeg_idct,
-              dim3((unsigned)std::min<int64_t>(65535, (nblk + 63) / 64)), dim3(64), 0, a,
-              reinterpret_cast<const int64_t*>(dev + o_bt), nblk);
-    HC_LAUNCH("k_jpeg_color", st, k_jpeg_color, dim3(64, (unsigned)nt), dim3(256), 0, a,
-              (const int64_t*)nullptr);
+              dim3((unsigned)std::min<int64_t>(65535, (nblk + 63) / 64)), dim3(64), 0, a, nblk);
+    HC_LAUNCH("k_jpeg_color", st, k_jpeg_color, dim3(64, (unsigned)nt), dim3(256), 0, a);
     return HC_OK;
   }();
-  // the work buffers are freed after the launches complete
+  // the staging and work buffers are reused by the next call: wait here
   if (rc == HC_OK) {
     e = hipStreamSynchronize(st);
     if (e != hipSuccess) rc = hip_check(e, "JPEG decode");
+  } else {
+    (void)hipStreamSynchronize(st);
   }
-  cleanup();
   return rc;
 }
 
