@@ -293,7 +293,10 @@ def bench_batch2d(steps, cpu_tiles):
            "pixels_per_s": px / (ms * 1e-3), "GBps_compulsory": 7 * px / (ms * 1e-3) / 1e9,
            "autotune": tune, "rows": plan.rows,
            "separate_calls_pixels_per_s": sep_px / (sep_ms * 1e-3),
-           "mismatched_pixels_vs_separate_calls_16_tiles": mism}
+           "mismatched_pixels_vs_separate_calls_16_tiles": mism,
+           "traffic_bytes_per_launch": read_traffic("k_tiles_ggs",
+                                                    "batch2d_512tiles_counters.txt"),
+           "compulsory_bytes_per_launch": 7 * px}
     if cpu_tiles > 0:
         import multiprocessing as mp
 
@@ -366,7 +369,9 @@ def bench_surrogate(steps, edge=512, ntubes=320):
            "surrogate_speedup": ms_plain / ms_sur,
            "surrogate_rel_l2": rel_l2(sur, plain),
            "surrogate_dice": dice(sur, plain, 0.5 * float(plain.max())),
-           "tflops_surrogate": 95904 * nvox / (ms_sur * 1e-3) / 1e12}
+           "tflops_surrogate": 95904 * nvox / (ms_sur * 1e-3) / 1e12,
+           "conv_traffic_bytes_per_launch": read_traffic("k_conv3d_mfma",
+                                                         "surrogate_512cubed_counters.txt")}
     del sur
     cs = CheckedSurrogate(model, shape, brick=64, tol=0.02)
     torch.cuda.synchronize()
