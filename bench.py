@@ -407,24 +407,37 @@ def bench_batch3d(steps, n=200, seed=77):
         vols.append(synthetic.tube_volume(s, ntubes=nt, seed=seed * 1000 + i, noise=0.08))
     res = {"volumes": n, "sigmas": [1, 2, 3]}
     weights = os.path.join(ROOT, "cucim_amd", "data", "frangi_surrogate_s123.npz")
-    gamma = None
     sb = None
-    if os.path.exists(weights):
-        from cucim_amd.surrogate import FrangiSurrogate, SurrogateBatch3D, dice, rel_l2
-
-        model = FrangiSurrogate.load(weights)
-        gamma = model.gamma  # plain and surrogate with the same fixed gamma
-        sb = SurrogateBatch3D(model, vols)
-    b = FrangiBatch3D(vols, sigmas=(1, 2, 3), gamma=gamma)
+    # the configuration's plain Frangi: gamma=None, i.e. gamma from each
+    # volume's own first-sigma Hessian norm
+    b = FrangiBatch3D(vols, sigmas=(1, 2, 3), gamma=None)
+    g1 = b.capture(streams=1)
+    g1.replay()
+    ms1 = _events_time(lambda i: g1.replay(), steps)
+    del g1
     g = b.capture()
     g.replay()
     ms = _events_time(lambda i: g.replay(), steps)
     lat = np.asarray(b.latencies_ms())
-    res.update({"voxels": b.voxels, "gamma": gamma if gamma is not None else "per volume",
+    res.update({"voxels": b.voxels, "gamma": "per volume",
                 "plain": {"ms_per_batch": ms, "voxels_per_s": b.voxels / (ms * 1e-3),
+                          "graph_branches": 8, "ms_per_batch_one_branch": ms1,
                           "latency_ms_p50": float(np.percentile(lat, 50)),
                           "latency_ms_p99": float(np.percentile(lat, 99)),
                           "eager_sum_ms": float(lat.sum())}})
+    if os.path.exists(weights):
+        from cucim_amd.surrogate import FrangiSurrogate, SurrogateBatch3D, dice, rel_l2
+
+        model = FrangiSurrogate.load(weights)
+        sb = SurrogateBatch3D(model, vols)
+        # the surrogate is trained for one fixed gamma: its comparison runs
+        # the plain schedule with that gamma
+        del g
+        b = FrangiBatch3D(vols, sigmas=(1, 2, 3), gamma=model.gamma)
+        g = b.capture()
+        g.replay()
+        ms = _events_time(lambda i: g.replay(), steps)
+        res["plain_fixed_gamma"] = {"gamma": model.gamma, "ms_per_batch": ms}
     if sb is not None:
         gs = sb.capture()
         gs.replay()

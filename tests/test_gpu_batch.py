@@ -158,3 +158,28 @@ def test_frangi_batch3d_graph_equals_eager(gpu):
             np.testing.assert_array_equal(gb, _bits(exp))
     lat = b.latencies_ms()
     assert len(lat) == len(shapes) and all(t > 0 for t in lat)
+
+
+@pytest.mark.parametrize("streams", [1, 3, 8])
+def test_frangi_batch3d_branches_equal_eager(gpu, streams):
+    """The batch captured as parallel graph branches (volume i on stream
+    i % streams): three replays, with eager Frangi calls and new inputs
+    between them, each equal to the eager result bit for bit."""
+    from cucim_amd import synthetic
+    from cucim_amd.batch import FrangiBatch3D
+    from cucim_amd.skimage.filters import frangi
+
+    shapes = [(48, 60, 52), (70, 49, 81), (55, 55, 55), (64, 96, 50), (50, 48, 64)]
+    vols = [synthetic.tube_volume(s, ntubes=3, seed=i, noise=0.08) for i, s in enumerate(shapes)]
+    b = FrangiBatch3D(vols, sigmas=(1, 2), gamma=None)
+    g = b.capture(streams=streams)
+    for rep in range(3):
+        for i, v in enumerate(b.volumes):
+            v.copy_(synthetic.tube_volume(tuple(v.shape), ntubes=3, seed=200 + 10 * rep + i,
+                                          noise=0.08))
+        exp = [_bits(frangi(v, sigmas=(1, 2), black_ridges=False)) for v in b.volumes]
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for o, e in zip(b.outputs, exp):
+            np.testing.assert_array_equal(_bits(o), e)
