@@ -26,7 +26,7 @@ MODE_REFLECT, MODE_MIRROR, MODE_NEAREST, MODE_WRAP, MODE_CONSTANT = range(5)
 ALGO_AUTO, ALGO_TILED, ALGO_GATHER = range(3)
 
 # hc_tune_knob
-TUNE_CONTIG_SEG, TUNE_SEP2D_TY, TUNE_HESSIAN_VALU, TUNE_EDGE_ROWS = range(4)
+TUNE_CONTIG_SEG, TUNE_SEP2D_TY, TUNE_HESSIAN_VALU, TUNE_EDGE_ROWS, TUNE_MFMA_WAVES = range(5)
 BIN_SUB, BIN_MUL, BIN_ADD = 0, 1, 2
 
 # hc_status

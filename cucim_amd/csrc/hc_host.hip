@@ -145,6 +145,8 @@ int hc_tune_set(int knob, int value) {
   if (knob == HC_TUNE_EDGE_ROWS && value != 0 && value != 8 && value != 16 && value != 32 &&
       value != 64 && value != 128)
     return hc::fail(HC_ERR_VALUE, "edge rows per wave must be 8, 16, 32, 64 or 128");
+  if (knob == HC_TUNE_MFMA_WAVES && value != 0 && value != 4 && value != 8)
+    return hc::fail(HC_ERR_VALUE, "matrix-core Hessian waves per work-group must be 4 or 8");
   hc::g_tune[knob].store(value);
   return HC_OK;
 }

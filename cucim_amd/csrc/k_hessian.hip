@@ -282,9 +282,15 @@ __device__ __forceinline__ int wave_id() {
   return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 }
 
-// Values per thread of one tile's interior (16 lines of N <= 64 NW, NW
-// waves per work-group) held in flight.
-constexpr int kMfmaNpf = 16;
+// Values per thread of one tile's interior held in flight: 16 lines of up to
+// 16 NW NBW positions (NBW blocks of 16 per wave) over 64 NW threads.
+__host__ __device__ constexpr int mfma_npf(int NBW) { return 4 * NBW; }
+
+// Waves per SIMD the compiler budgets registers for: four blocks per wave
+// need the larger share (the chain pass's two first-op results, the final
+// pass's six Hessian inputs); two or one leave room for more waves.
+__host__ __device__ constexpr int chain_wpe(int NBW) { return NBW >= 4 ? 5 : 6; }
+__host__ __device__ constexpr int final_wpe(int NBW) { return NBW >= 4 ? 4 : 6; }
 
 // Blocks b = wave + NW i (i < NBW) of 16 positions: acc[s][i] for the NS
 // weight sets wp[s] (zero-padded: trimmed tap t at wp[15 + t]).  The A
@@ -429,9 +435,9 @@ __device__ __forceinline__ void fill_halo(float* X, float* Y, int E, int N, int 
 constexpr int kChainSPW = 1;
 
 template <int NBW, int NW>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(5))) void k_chain_mfma(
-    const ChainArgs<float> a) {
-  constexpr int NPF = kMfmaNpf, NT = 64 * NW;
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(chain_wpe(NBW)))) void
+k_chain_mfma(const ChainArgs<float> a) {
+  constexpr int NPF = mfma_npf(NBW), NT = 64 * NW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int E = a.E, Nr = a.Nr, Km = a.Km, shift = a.shift;
   const int nd = (Km + 30) / 16;  // input blocks per output block
@@ -933,7 +939,7 @@ struct FinalGeom {
 // zeros.  Every load is issued before any is used (the index of a masked
 // one is clamped into the row and its value replaced afterwards).  Returns
 // whether a staged value is not finite.
-template <int NW>
+template <int NBW, int NW>
 __device__ __forceinline__ bool final_stage(const FinalArgs<float>& a, const FinalGeom& g,
                                             float* X, int LS, int J, int64_t tile, int tid) {
   typedef float F4 __attribute__((ext_vector_type(4)));
@@ -946,15 +952,15 @@ __device__ __forceinline__ bool final_stage(const FinalArgs<float>& a, const Fin
   float* xl = X + l * LS + g.shift;
   bool bad = false;
   if (a.vec4) {
-    // N <= 64 NW, so a line has at most 4 CS chunks
-    F4 v[4];
+    // N <= 16 NW NBW, so a line has at most NBW CS chunks
+    F4 v[NBW];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < NBW; ++k) {
       const int c = c0 + CS * k;
       v[k] = reinterpret_cast<const F4*>(rp)[4 * c < n ? c : 0];
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < NBW; ++k) {
       const int c = c0 + CS * k;
       if (4 * c >= n) continue;
       const F4 w = rok ? v[k] : F4{0.f, 0.f, 0.f, 0.f};
@@ -963,14 +969,14 @@ __device__ __forceinline__ bool final_stage(const FinalArgs<float>& a, const Fin
       *reinterpret_cast<F2*>(xl + 4 * c + 2) = F2{w[2], w[3]};
     }
   } else {
-    float v[16];
+    float v[4 * NBW];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
+    for (int k = 0; k < 4 * NBW; ++k) {
       const int q = c0 + CS * k;
       v[k] = rp[q < n ? q : 0];
     }
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
+    for (int k = 0; k < 4 * NBW; ++k) {
       const int q = c0 + CS * k;
       if (q >= n) continue;
       const float w = rok ? v[k] : 0.f;
@@ -992,7 +998,7 @@ __device__ __forceinline__ void final_input_mfma(const FinalArgs<float>& a, cons
   const int tid = threadIdx.x, lane = tid & 63, wv = wave_id();
   const int kk = lane >> 4, ii = lane & 15;
   constexpr int NT = 64 * NW;
-  const bool bad = final_stage<NW>(a, g, xin, LS, J, tile, tid);
+  const bool bad = final_stage<NBW, NW>(a, g, xin, LS, J, tile, tid);
   const bool skip0 = __syncthreads_or(bad) || !finite(a.cval);
   fill_halo(xin, nullptr, E, (int)g.N, shift, hsrc, a.cval, tid, NT, 1, LS);
   __syncthreads();
@@ -1040,8 +1046,8 @@ __device__ __forceinline__ void final_all_mfma(const FinalArgs<float>& a, const 
 // 1 Frangi storing eigenvalues and reducing max r_g (first sigma, gamma
 // unknown), 2 Frangi vesselness into the running maximum.
 template <int NIN, int NBW, int NW, int EP>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void k_final_mfma(
-    const FinalArgs<float> a) {
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(final_wpe(NBW)))) void
+k_final_mfma(const FinalArgs<float> a) {
   constexpr int NT = 64 * NW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   FinalGeom g;
@@ -1316,9 +1322,14 @@ inline bool mfma_geom(const TapPack<float>& S, const TapPack<float>& D, int64_t 
 }
 
 // Waves per work-group of the matrix-core passes: 4 for lines of at most
-// 256, 8 up to 512 (each thread stages kMfmaNpf values of a tile's interior,
-// and a wave holds at most 4 blocks of 16 outputs).
-inline int mfma_waves(int64_t N) { return N <= 64 * 4 ? 4 : 8; }
+// 256, 8 up to 512 (a wave holds at most 4 blocks of 16 outputs); the
+// HC_TUNE_MFMA_WAVES knob set to 8 spreads lines of 129..256 over 8 waves
+// of 2 blocks each (fewer accumulators per wave, more waves per CU for the
+// same LDS).
+inline int mfma_waves(int64_t N) {
+  if (N > 64 * 4) return 8;
+  return tune_get(HC_TUNE_MFMA_WAVES) == 8 && N > 128 ? 8 : 4;
+}
 
 template <int NBW, int NW>
 int launch_chain_mfma_nbw(ChainArgs<float>& a, const MfmaGeomHost& g, hipStream_t st) {
@@ -1346,7 +1357,8 @@ inline int launch_chain_mfma(ChainArgs<float>& a, hipStream_t st) {
   a.shift = g.shift;
   a.Nr = g.Nr;
   a.E = g.E;
-  if (mfma_waves(a.N) == 8) return launch_chain_mfma_nbw<4, 8>(a, g, st);
+  if (mfma_waves(a.N) == 8)
+    return nblk <= 16 ? launch_chain_mfma_nbw<2, 8>(a, g, st) : launch_chain_mfma_nbw<4, 8>(a, g, st);
   if (nblk <= 4) return launch_chain_mfma_nbw<1, 4>(a, g, st);
   if (nblk <= 8) return launch_chain_mfma_nbw<2, 4>(a, g, st);
   return launch_chain_mfma_nbw<4, 4>(a, g, st);
@@ -1463,7 +1475,9 @@ int launch_final_mfma(const FinalArgs<float>& a0, hipStream_t st) {
   // rows of a multiple of 4 values from 16-byte aligned arrays: 16-byte loads
   a.vec4 = a.N % 4 == 0;
   for (int j = 0; j < NIN; ++j) a.vec4 &= ((uintptr_t)a.in[j] & 15) == 0;
-  if (mfma_waves(a.N) == 8) return launch_final_mfma_nbw<NIN, 4, 8>(a, g, st);
+  if (mfma_waves(a.N) == 8)
+    return nblk <= 16 ? launch_final_mfma_nbw<NIN, 2, 8>(a, g, st)
+                      : launch_final_mfma_nbw<NIN, 4, 8>(a, g, st);
   if (nblk <= 4) return launch_final_mfma_nbw<NIN, 1, 4>(a, g, st);
   if (nblk <= 8) return launch_final_mfma_nbw<NIN, 2, 4>(a, g, st);
   return launch_final_mfma_nbw<NIN, 4, 4>(a, g, st);

@@ -116,6 +116,8 @@ typedef enum hc_tune_knob {
   HC_TUNE_EDGE_ROWS = 3,    /* fused 2-D edge magnitude (float input of the
                                output's type): output rows per wave, 8, 16,
                                32, 64 or 128 */
+  HC_TUNE_MFMA_WAVES = 4,   /* matrix-core Hessian passes, lines of 129..256:
+                               waves per work-group, 4 or 8 */
   HC_TUNE_NKNOBS = 8
 } hc_tune_knob;
 int hc_tune_set(int knob, int value);
