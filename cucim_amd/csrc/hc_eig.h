@@ -216,6 +216,35 @@ __device__ __forceinline__ F frangi_v3r(F l1, F l2, F l3, F r_g, const FrangiNeg
   return result;
 }
 
+// float32 forms of the two above, used by the fused float32 passes: the
+// exponentials in float (1 - e^x as -expm1(x), accurate where x is small)
+// instead of double.  The result then differs from the reference's
+// double-then-round value by a few float ulps (each exponential within about
+// an ulp of its argument's relative error, one float rounding per product),
+// far below the rounding of the float32 Hessian the vesselness is built
+// from; it saves the three double exponentials per voxel and sigma.
+template <>
+__device__ __forceinline__ float frangi_v2r<float>(float l1, float l2, float r_g,
+                                                   const FrangiNegRecip& q) {
+  float r_b = fabsf(l1) / fmaxf(l2, 1.0e-10f);
+  r_b *= r_b;
+  const float result = expf(r_b * (float)q.b);
+  return result * -expm1f(r_g * (float)q.g);
+}
+
+template <>
+__device__ __forceinline__ float frangi_v3r<float>(float l1, float l2, float l3, float r_g,
+                                                   const FrangiNegRecip& q) {
+  const float lam2 = fmaxf(l2, 1.0e-10f);
+  const float lam3 = fmaxf(l3, 1.0e-10f);
+  float r_a = lam2 / lam3;
+  r_a *= r_a;
+  const float r_b = (l1 * l1) / (lam2 * lam3);
+  float result = -expm1f(r_a * (float)q.a);
+  result *= expf(r_b * (float)q.b);
+  return result * -expm1f(r_g * (float)q.g);
+}
+
 // cp.maximum: NaN-propagating
 template <typename F>
 __device__ __forceinline__ F nanmax(F a, F b) {
