@@ -833,19 +833,43 @@ __host__ __device__ inline void ycc_rgb(int Y, int Cb, int Cr, uint8_t* o) {
 // Decoded tile -> region: planes[c] hold each component at its own
 // resolution (pw = bw * 8 wide); pixel (tx, ty) of the tile goes to region
 // pixel (tx + x0 - rx, ty + y0 - ry) when it lies inside the region.
-__host__ __device__ void jpeg_pixel(const JTile& jt, uint8_t* const* planes, int tx, int ty,
-                                    uint8_t* dst, int nch) {
-  int v[3];
-  for (int c = 0; c < jt.ncomp; ++c) {
-    const JComp& k = jt.comp[c];
-    const int fx = jt.hmax / k.h, fy = jt.vmax / k.v;
-    const int dw = (jt.width * k.h + jt.hmax - 1) / jt.hmax;
-    const int dh = (jt.height * k.v + jt.vmax - 1) / jt.vmax;
-    v[c] = up_sample(planes[c], k.bw * 8, dw, dh, fx, fy, tx, ty);
+// What the per-pixel colour stage needs of a decoded tile, in registers:
+// each component's plane, row stride, own dimensions and upsampling factors.
+struct PixCtx {
+  const uint8_t* pl[3];
+  int stride[3], dw[3], dh[3], fx[3], fy[3];
+  int ncomp, transform;
+};
+
+__host__ __device__ inline PixCtx pix_ctx(const JTile& jt, uint8_t* const* planes) {
+  PixCtx p;
+  p.ncomp = jt.ncomp;
+  p.transform = jt.transform;
+  for (int c = 0; c < 3; ++c) {
+    const int cc = c < jt.ncomp ? c : 0;
+    const JComp& k = jt.comp[cc];
+    p.pl[c] = planes[cc];
+    p.stride[c] = k.bw * 8;
+    p.fx[c] = jt.hmax / k.h;
+    p.fy[c] = jt.vmax / k.v;
+    p.dw[c] = (jt.width * k.h + jt.hmax - 1) / jt.hmax;
+    p.dh[c] = (jt.height * k.v + jt.vmax - 1) / jt.vmax;
   }
-  if (jt.ncomp == 1) {
+  return p;
+}
+
+// Tile pixel (tx, ty) into dst (nch samples).
+__host__ __device__ inline void jpeg_pixel(const PixCtx& p, int tx, int ty, uint8_t* dst,
+                                           int nch) {
+  int v[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    v[c] = c < p.ncomp ? up_sample(p.pl[c], p.stride[c], p.dw[c], p.dh[c], p.fx[c], p.fy[c],
+                                   tx, ty)
+                       : 0;
+  if (p.ncomp == 1) {
     for (int c = 0; c < nch; ++c) dst[c] = (uint8_t)v[0];
-  } else if (jt.transform) {
+  } else if (p.transform) {
     ycc_rgb(v[0], v[1], v[2], dst);
   } else {
     dst[0] = (uint8_t)v[0];
@@ -1021,15 +1045,50 @@ __device__ JState jseg_run(const JSeg& sg, JState st, int64_t end_bit, int& g, i
   return JState{tell(), u, z};
 }
 
-__global__ __launch_bounds__(64) void k_jpeg_entropy_lds(const JDev a, int cap) {
+// Lanes of the entropy kernel's work-group: one tile's segment is split into
+// kJLanes parts.
+constexpr int kJWaves = 4, kJLanes = 64 * kJWaves;
+
+__device__ inline int wave_incl_scan(int v, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+// Exclusive prefix sum of v over the work-group's kJLanes threads (total in
+// *total); sh holds kJWaves ints.  Contains two barriers.
+__device__ inline int block_excl_scan(int v, int* sh, int* total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int incl = wave_incl_scan(v, lane);
+  if (lane == 63) sh[w] = incl;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kJWaves; ++k) {
+    const int t = sh[k];
+    off += k < w ? t : 0;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + incl - v;
+}
+
+__global__ __launch_bounds__(kJLanes) void k_jpeg_entropy_lds(const JDev a, int cap) {
   extern __shared__ __align__(16) uint8_t smem[];
+  __shared__ int scan_sh[kJWaves];
+  __shared__ int64_t end_bit[kJLanes];
+  __shared__ int end_u[kJLanes], end_z[kJLanes];
   const int t = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
   const JTile& jt = a.tiles[t];
   int16_t* coef = a.coef + jt.coef0 * 64;
   const int64_t n = jt.data_len;
   if (jt.restart != 0 || n > cap || n < 256) {
-    if (lane == 0) jpeg_entropy(jt, a.tabsets[jt.tabset], a.data, coef);
+    if (tid == 0) jpeg_entropy(jt, a.tabsets[jt.tabset], a.data, coef);
     return;
   }
   // tables: dc[0], dc[1], ac[0], ac[1] as 32-bit words
@@ -1040,17 +1099,17 @@ __global__ __launch_bounds__(64) void k_jpeg_entropy_lds(const JDev a, int cap) 
     const uint32_t* src1 = reinterpret_cast<const uint32_t*>(&tb.ac[0]);
     uint32_t* dst = reinterpret_cast<uint32_t*>(smem);
     constexpr int half = 2 * (int)sizeof(HuffTab) / 4;
-    for (int i = lane; i < half; i += 64) {
+    for (int i = tid; i < half; i += kJLanes) {
       dst[i] = src0[i];
       dst[half + i] = src1[i];
     }
   }
-  // destuff the segment into LDS
+  // destuff the segment into LDS, 16 bytes per thread per round
   uint8_t* ds = smem + kTabBytes;
   const uint8_t* g = a.data + jt.data_off;
   int out = 0;
-  for (int64_t base = 0; base < n; base += 64 * 16) {
-    const int64_t i0 = base + lane * 16;
+  for (int64_t base = 0; base < n; base += kJLanes * 16) {
+    const int64_t i0 = base + tid * 16;
     uint8_t b[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) b[k] = i0 + k < n ? g[i0 + k] : 0;
@@ -1061,28 +1120,21 @@ __global__ __launch_bounds__(64) void k_jpeg_entropy_lds(const JDev a, int cap) 
       const uint8_t pv = k ? b[k - 1] : before;
       if (i0 + k < n && !(pv == 0xFF && b[k] == 0x00)) keep |= 1u << k;
     }
-    const int cnt = __popc(keep);
-    // inclusive wave scan of cnt
-    int incl = cnt;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int v = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += v;
-    }
-    int pos = out + incl - cnt;
+    int total;
+    int pos = out + block_excl_scan(__popc(keep), scan_sh, &total);
 #pragma unroll
     for (int k = 0; k < 16; ++k)
       if (keep & (1u << k)) ds[pos++] = b[k];
-    out += __shfl(incl, 63, 64);
+    out += total;
   }
   __shared__ uint8_t zz[64];
   __shared__ int8_t ucomp[12], ubx[12], uby[12];
-  zz[lane] = (uint8_t)kZigzag[lane];
-  if (lane < 8) ds[out + lane] = 0;  // the refill's second word past the end
+  if (tid < 64) zz[tid] = (uint8_t)kZigzag[tid];
+  if (tid < 8) ds[out + tid] = 0;  // the refill's second word past the end
   const int ncomp = jt.ncomp;
   int U = 0;
   for (int c = 0; c < ncomp; ++c) U += jt.comp[c].h * jt.comp[c].v;
-  if (lane == 0) {  // units of an MCU: components in frame order, h x v blocks each
+  if (tid == 0) {  // units of an MCU: components in frame order, h x v blocks each
     int u = 0;
     for (int c = 0; c < ncomp; ++c)
       for (int by = 0; by < jt.comp[c].v; ++by)
@@ -1114,10 +1166,10 @@ __global__ __launch_bounds__(64) void k_jpeg_entropy_lds(const JDev a, int cap) 
     seg.cbw[c] = jt.comp[cc].bw;
     seg.cb0[c] = jt.comp[cc].block0;
   }
-  // each lane owns S bits (a multiple of 32) of the destuffed segment
+  // each thread owns S bits (a multiple of 32) of the destuffed segment
   const int64_t nbits = (int64_t)out * 8;
-  const int64_t S = ((nbits + 63) / 64 + 31) / 32 * 32;
-  const int64_t b0 = std::min<int64_t>((int64_t)lane * S, nbits);
+  const int64_t S = ((nbits + kJLanes - 1) / kJLanes + 31) / 32 * 32;
+  const int64_t b0 = std::min<int64_t>((int64_t)tid * S, nbits);
   const int64_t b1 = std::min<int64_t>(b0 + S, nbits);
   // A: speculative decode of the own part from an assumed MCU start
   JState st{b0, 0, 0};
@@ -1125,14 +1177,15 @@ __global__ __launch_bounds__(64) void k_jpeg_entropy_lds(const JDev a, int cap) 
   int nblk = 0;
   JState en = jseg_run<false>(seg, st, b1, gb, nblk);
   // B: take the left neighbour's end state as the own start until no end
-  // state changes (lane 0's start is exact, so lane k is exact after k
+  // state changes (thread 0's start is exact, so thread k is exact after k
   // rounds; decoding self-synchronises, so a few rounds suffice)
-  for (int round = 0; round < 64; ++round) {
-    JState ps;
-    ps.bit = __shfl_up(en.bit, 1, 64);
-    ps.u = __shfl_up(en.u, 1, 64);
-    ps.z = __shfl_up(en.z, 1, 64);
-    if (lane == 0) ps = JState{0, 0, 0};
+  for (int round = 0; round < kJLanes; ++round) {
+    end_bit[tid] = en.bit;
+    end_u[tid] = en.u;
+    end_z[tid] = en.z;
+    __syncthreads();
+    JState ps{0, 0, 0};
+    if (tid > 0) ps = JState{end_bit[tid - 1], end_u[tid - 1], end_z[tid - 1]};
     bool changed = false;
     if (ps.bit != st.bit || ps.u != st.u || ps.z != st.z) {
       st = ps;
@@ -1142,16 +1195,11 @@ __global__ __launch_bounds__(64) void k_jpeg_entropy_lds(const JDev a, int cap) 
       changed = e2.bit != en.bit || e2.u != en.u || e2.z != en.z;
       en = e2;
     }
-    if (!__any(changed)) break;
+    if (!__syncthreads_or(changed)) break;
   }
   // C: first block of each part = blocks completed by the parts before it
-  int incl = nblk;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int v = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += v;
-  }
-  gb = incl - nblk;
+  int tot;
+  gb = block_excl_scan(nblk, scan_sh, &tot);
   // D: decode again, storing coefficients (DC as differences)
   nblk = 0;
   jseg_run<true>(seg, st, b1, gb, nblk);
@@ -1163,17 +1211,11 @@ __global__ __launch_bounds__(64) void k_jpeg_entropy_lds(const JDev a, int cap) 
   for (int c = 0; c < ncomp; ++c) {
     const int hv = seg.ch[c] * seg.cv[c];
     const int N = nmcu * hv;
-    const int per = (N + 63) / 64;
-    const int k0 = min(lane * per, N), k1 = min(k0 + per, N);
+    const int per = (N + kJLanes - 1) / kJLanes;
+    const int k0 = min(tid * per, N), k1 = min(k0 + per, N);
     int sum = 0;
     for (int k = k0; k < k1; ++k) sum += *jseg_dc(seg, c, hv, k);
-    int inc = sum;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int v = __shfl_up(inc, o, 64);
-      if (lane >= o) inc += v;
-    }
-    int run = inc - sum;
+    int run = block_excl_scan(sum, scan_sh, &tot);
     for (int k = k0; k < k1; ++k) {
       int16_t* p = jseg_dc(seg, c, hv, k);
       run += *p;
@@ -1216,20 +1258,26 @@ __global__ __launch_bounds__(64) void k_jpeg_idct(const JDev a, int64_t nblk) {
 __global__ __launch_bounds__(256) void k_jpeg_color(const JDev a) {
   const int t = blockIdx.y;
   const JTile& jt = a.tiles[t];
-  // the tile's overlap with the region
+  // the tile's overlap with the region (tile and region offsets fit 32 bits
+  // relative to each other: a tile spans at most 65535 pixels)
   const int64_t ox0 = std::max<int64_t>(jt.x0, a.rx), oy0 = std::max<int64_t>(jt.y0, a.ry);
   const int64_t ox1 = std::min<int64_t>(jt.x0 + jt.vis_w, a.rx + a.rw);
   const int64_t oy1 = std::min<int64_t>(jt.y0 + jt.vis_h, a.ry + a.rh);
-  const int64_t ow = ox1 - ox0, oh = oy1 - oy0;
-  if (ow <= 0 || oh <= 0) return;
+  if (ox1 <= ox0 || oy1 <= oy0) return;
+  const uint32_t ow = (uint32_t)(ox1 - ox0), n = ow * (uint32_t)(oy1 - oy0);
   uint8_t* planes[3];
-  for (int c = 0; c < jt.ncomp; ++c) planes[c] = a.planes + (jt.coef0 + jt.comp[c].block0) * 64;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < ow * oh;
-       i += (int64_t)gridDim.x * 256) {
-    const int64_t yy = i / ow, xx = i - yy * ow;
-    const int64_t gx = ox0 + xx, gy = oy0 + yy;
-    jpeg_pixel(jt, planes, (int)(gx - jt.x0), (int)(gy - jt.y0),
-               a.out + ((gy - a.ry) * a.rw + (gx - a.rx)) * a.nch, a.nch);
+  for (int c = 0; c < 3; ++c)
+    planes[c] = a.planes + (jt.coef0 + jt.comp[c < jt.ncomp ? c : 0].block0) * 64;
+  // everything the loop reads of the tile, in registers (the byte stores
+  // below could otherwise alias the tile table and force reloads)
+  const PixCtx pc = pix_ctx(jt, planes);
+  const int tx0 = (int)(ox0 - jt.x0), ty0 = (int)(oy0 - jt.y0);
+  const int nch = a.nch;
+  uint8_t* __restrict__ out = a.out + ((oy0 - a.ry) * a.rw + (ox0 - a.rx)) * nch;
+  const int64_t orow = a.rw * nch;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const uint32_t yy = i / ow, xx = i - yy * ow;
+    jpeg_pixel(pc, tx0 + (int)xx, ty0 + (int)yy, out + yy * orow + xx * nch, nch);
   }
 }
 
@@ -1333,11 +1381,12 @@ int decode_jpeg_host(const Ifd& f, const std::vector<uint8_t>& src, int nch,
   }
   uint8_t* pl[3] = {nullptr, nullptr, nullptr};
   for (int c = 0; c < jt.ncomp; ++c) pl[c] = planes.data() + jt.comp[c].block0 * 64;
+  const PixCtx pc = pix_ctx(jt, pl);
   px.assign((size_t)(f.tile_w * f.tile_h * nch), 0);
   const int64_t w = std::min<int64_t>(jt.width, f.tile_w), h = std::min<int64_t>(jt.height, f.tile_h);
   for (int64_t y = 0; y < h; ++y)
     for (int64_t x = 0; x < w; ++x)
-      jpeg_pixel(jt, pl, (int)x, (int)y, px.data() + (y * f.tile_w + x) * nch, nch);
+      jpeg_pixel(pc, (int)x, (int)y, px.data() + (y * f.tile_w + x) * nch, nch);
   return HC_OK;
 }
 
@@ -1632,7 +1681,7 @@ int hc_wsi_read_region_device(const hc_wsi* h, int level, int64_t x, int64_t y, 
   int64_t need = 0;
   for (int i = 0; i < nt; ++i)
     if (T[i].restart == 0 && T[i].data_len <= kMaxStage) need = std::max(need, T[i].data_len);
-  const int cap = (int)((need + 16 * 64 - 1) / (16 * 64) * (16 * 64));
+  const int cap = (int)((need + 16 * kJLanes - 1) / (16 * kJLanes) * (16 * kJLanes));
   // device work buffer: the staged bytes at the same offsets, then the
   // coefficients and the component planes
   const int64_t o_coef = al(o_blob + boff[nt] + 16);
@@ -1686,8 +1735,8 @@ int hc_wsi_read_region_device(const hc_wsi* h, int level, int64_t x, int64_t y, 
                                                 (int)smem);
       if (fe != hipSuccess) return hip_check(fe, "hipFuncSetAttribute(k_jpeg_entropy_lds)");
     }
-    HC_LAUNCH("k_jpeg_entropy_lds", st, k_jpeg_entropy_lds, dim3((unsigned)nt), dim3(64), smem,
-              a, cap);
+    HC_LAUNCH("k_jpeg_entropy_lds", st, k_jpeg_entropy_lds, dim3((unsigned)nt), dim3(kJLanes),
+              smem, a, cap);
     HC_LAUNCH("k_jpeg_idct", st, k_jpeg_idct,
               dim3((unsigned)std::min<int64_t>(65535, (nblk + 63) / 64)), dim3(64), 0, a, nblk);
     HC_LAUNCH("k_jpeg_color", st, k_jpeg_color, dim3(64, (unsigned)nt), dim3(256), 0, a);
