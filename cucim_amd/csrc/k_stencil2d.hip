@@ -185,11 +185,26 @@ struct GgsArgs {
 typedef const __attribute__((address_space(1))) uint8_t* gu8_ptr;
 typedef __attribute__((address_space(1))) float* gf32_ptr;
 
-__device__ __forceinline__ float gray_u8(gu8_ptr p, float scale) {
-  const double r = (double)((float)p[0] * scale);
-  const double g = (double)((float)p[1] * scale);
-  const double b = (double)((float)p[2] * scale);
-  return (float)add_rn(add_rn(mul_rn(0.2125, r), mul_rn(0.7154, g)), mul_rn(0.0721, b));
+// gray = float32(0.2125 r' + 0.7154 g' + 0.0721 b'), products and sums in
+// double, r' = float32(r * float32(1/255)) (colorconv.py:1037-1095 after
+// img_as_float).  Each double product depends on one byte only: tables of the
+// 256 products per channel, filled per work-group in LDS, give the same bits
+// with two double additions per pixel.
+struct GrayLut {
+  double r[256], g[256], b[256];
+};
+
+__device__ __forceinline__ void gray_lut_fill(GrayLut* t, float scale) {
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+    const double v = (double)((float)i * scale);
+    t->r[i] = mul_rn(0.2125, v);
+    t->g[i] = mul_rn(0.7154, v);
+    t->b[i] = mul_rn(0.0721, v);
+  }
+}
+
+__device__ __forceinline__ float gray_lut(gu8_ptr p, const GrayLut* t) {
+  return (float)add_rn(add_rn(t->r[p[0]], t->g[p[1]]), t->b[p[2]]);
 }
 
 // one tap with the reference's zero-weight skip
@@ -202,8 +217,13 @@ __device__ __forceinline__ float etap(float acc, float x, float w) {
 template <int K, int TY>
 __global__ __launch_bounds__(256) void k_tiles_ggs(const GgsArgs a, const RegTaps<K> tg) {
   constexpr int GR = TY + 2;  // gaussian rows with the edge halo
+  // V: the vertical pass's output; the horizontal pass overwrites it in
+  // place with G (the gaussian with its edge halo)
   __shared__ __align__(16) float V[GR * kRegUStride];
-  __shared__ __align__(16) float G[GR * kRegUStride];
+  float* const G = V;
+  __shared__ GrayLut glut;
+  gray_lut_fill(&glut, a.scale);
+  __syncthreads();
   const int GX = a.TX + 2;
   const int c = threadIdx.x;
   const int64_t G_ = gridDim.x;
@@ -228,7 +248,7 @@ __global__ __launch_bounds__(256) void k_tiles_ggs(const GgsArgs a, const RegTap
         // mapping
         const gu8_ptr col = in + 3 * (ya * W + (xr >= 0 ? xr : 0));
 #pragma unroll
-        for (int j = 0; j < GR + K - 1; ++j) v[j] = gray_u8(col + 3 * W * j, a.scale);
+        for (int j = 0; j < GR + K - 1; ++j) v[j] = gray_lut(col + 3 * W * j, &glut);
         if (xr < 0) {
 #pragma unroll
           for (int j = 0; j < GR + K - 1; ++j) v[j] = a.gcval;
@@ -237,7 +257,7 @@ __global__ __launch_bounds__(256) void k_tiles_ggs(const GgsArgs a, const RegTap
 #pragma unroll
         for (int j = 0; j < GR + K - 1; ++j) {
           const int64_t yr = remap(ya + j, H, a.gmode);
-          v[j] = (yr >= 0 && xr >= 0) ? gray_u8(in + 3 * (yr * W + xr), a.scale) : a.gcval;
+          v[j] = (yr >= 0 && xr >= 0) ? gray_lut(in + 3 * (yr * W + xr), &glut) : a.gcval;
         }
       }
       float acc[GR];
@@ -246,15 +266,28 @@ __global__ __launch_bounds__(256) void k_tiles_ggs(const GgsArgs a, const RegTap
       for (int i = 0; i < GR; ++i) V[i * kRegUStride + c] = xr < 0 ? a.gcval : acc[i];
     }
     __syncthreads();
-    // 2. horizontal gaussian pass into G
+    // 2. horizontal gaussian pass into G, in place: every item's inputs are
+    // read before any item's outputs are written
     {
-      const int nq = GX / kRegP;
-      for (int q = threadIdx.x; q < GR * nq; q += 256) {
-        const int i = q / nq, o = (q - i * nq) * kRegP;
-        float r[kRegP];
-        reg_hpass<K>(r, V + i * kRegUStride + o, tg);
+      constexpr int nq = reg_strip_outputs(K) / kRegP, NQ = (GR * nq + 255) / 256;
+      float r[NQ][kRegP];
 #pragma unroll
-        for (int j = 0; j < kRegP; ++j) G[i * kRegUStride + o + j] = r[j];
+      for (int m = 0; m < NQ; ++m) {
+        const int q = threadIdx.x + 256 * m;
+        if (q < GR * nq) {
+          const int i = q / nq, o = (q - i * nq) * kRegP;
+          reg_hpass<K>(r[m], V + i * kRegUStride + o, tg);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < NQ; ++m) {
+        const int q = threadIdx.x + 256 * m;
+        if (q < GR * nq) {
+          const int i = q / nq, o = (q - i * nq) * kRegP;
+#pragma unroll
+          for (int j = 0; j < kRegP; ++j) G[i * kRegUStride + o + j] = r[m][j];
+        }
       }
     }
     __syncthreads();
