@@ -75,6 +75,9 @@ ENTRY_POINTS = (
     "hc_wsi_properties",
     "hc_wsi_read_region",
     "hc_wsi_read_region_device",
+    "hc_wsi_associated_count",
+    "hc_wsi_associated_info",
+    "hc_wsi_associated_read",
     "hc_sqrt_inplace",
     "hc_copy_cast",
     "hc_batch2d_plan",
@@ -173,6 +176,9 @@ _SIGNATURES = {
                             ctypes.c_int64, _v, _i], _i),
     "hc_wsi_read_region_device": ([_v, _i, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                    ctypes.c_int64, _v, _i, _v], _i),
+    "hc_wsi_associated_count": ([_v], _i),
+    "hc_wsi_associated_info": ([_v, _i, ctypes.c_char_p, _i, _P(ctypes.c_int64)], _i),
+    "hc_wsi_associated_read": ([_v, _i, _v, _i], _i),
     "hc_sqrt_inplace": ([_A, _v], _i),
     "hc_copy_cast": ([_A, _A, _v], _i),
     "hc_batch2d_plan": ([_v, _i, _i, _i, _v, ctypes.c_int64], ctypes.c_int64),

@@ -63,6 +63,11 @@ class CuImage:
         self._vendor = vendor.value.decode()
         # level_downsamples: mean of the width and height ratios in float32
         # (cuslide.cpp:227-234)
+        self._assoc = {}
+        name = ctypes.create_string_buffer(64)
+        for k in range(lib.hc_wsi_associated_count(h)):
+            _lib.check(lib.hc_wsi_associated_info(h, k, name, len(name), info), "CuImage")
+            self._assoc[name.value.decode()] = (k, int(info[0]), int(info[1]), int(info[2]))
         w0, h0 = np.float32(self._levels[0][0]), np.float32(self._levels[0][1])
         self._downsamples = [
             float((w0 / np.float32(L[0]) + h0 / np.float32(L[1])) / np.float32(2))
@@ -150,6 +155,7 @@ class CuImage:
                 "spacing_units": self.spacing_units(),
                 "resolutions": {k: list(v) if isinstance(v, tuple) else v
                                 for k, v in self.resolutions.items()},
+                "associated_images": sorted(self._assoc),
             },
             "tiff": {
                 "vendor": self._vendor,
@@ -160,6 +166,32 @@ class CuImage:
                 "compression": [_COMPRESSION.get(L[5], str(L[5])) for L in self._levels],
             },
         }
+
+    @property
+    def associated_images(self):
+        """Names of the associated images (Aperio: ``thumbnail``, ``label``,
+        ``macro``); an empty set for generic TIFF files."""
+        return set(self._assoc)
+
+    def associated_image(self, name, device="cpu", num_workers=0):
+        """The whole associated image ``name`` as ``(height, width,
+        channels)`` uint8: a numpy array, or a torch tensor in device memory
+        for ``device='cuda'`` (decoded on the host, then uploaded)."""
+        if name not in self._assoc:
+            raise ValueError(f"no associated image named {name!r}; the file has "
+                             f"{sorted(self._assoc)}")
+        k, w, h, c = self._assoc[name]
+        out = np.empty((h, w, c), np.uint8)
+        _lib.check(_lib.load_library().hc_wsi_associated_read(
+            self._h, k, out.ctypes.data_as(ctypes.c_void_p), int(num_workers)),
+            "associated_image")
+        if _is_cuda(device):
+            import torch
+
+            from .. import _array
+
+            return torch.from_numpy(out).to(_array.device())
+        return out
 
     @property
     def background(self):

@@ -56,6 +56,7 @@ struct Wsi {
   bool big = false, le = true;
   std::vector<Ifd> ifds;
   std::vector<int> levels;  // IFD of each pyramid level
+  std::vector<std::pair<std::string, int>> assoc;  // associated image name, IFD
   uint8_t background = 0;
   std::string vendor = "generic";
   ~Wsi() {
@@ -222,6 +223,20 @@ int parse_tiff(Wsi& w) {
   // first
   for (int i = 0; i < (int)w.ifds.size(); ++i)
     if (i == 0 || w.vendor == "generic" || w.ifds[i].tiled) w.levels.push_back(i);
+  // associated images of Aperio files (tiff.cpp:670-700): an untiled IFD 1
+  // of subfile type 0 is the thumbnail, subfile type 1 the label, 9 the
+  // macro image.  (Philips files name theirs in the XML description, which
+  // is not read here.)
+  if (w.vendor == "aperio")
+    for (int i = 1; i < (int)w.ifds.size(); ++i) {
+      const Ifd& f = w.ifds[i];
+      if (f.tiled) continue;
+      const char* name = nullptr;
+      if (i == 1 && f.subfile == 0) name = "thumbnail";
+      else if (f.subfile == 1) name = "label";
+      else if (f.subfile == 9) name = "macro";
+      if (name) w.assoc.emplace_back(name, i);
+    }
   std::stable_sort(w.levels.begin(), w.levels.end(), [&](int a, int b) {
     const Ifd &fa = w.ifds[a], &fb = w.ifds[b];
     return fa.width != fb.width ? fa.width > fb.width : fa.height > fb.height;
@@ -1390,9 +1405,8 @@ int decode_jpeg_host(const Ifd& f, const std::vector<uint8_t>& src, int nch,
   return HC_OK;
 }
 
-int read_region_host(const Wsi& w, int level, int64_t x, int64_t y, int64_t rw, int64_t rh,
+int read_region_host(const Wsi& w, const Ifd& f, int64_t x, int64_t y, int64_t rw, int64_t rh,
                      uint8_t* out, int workers) {
-  const Ifd& f = w.ifds[w.levels[level]];
   int rc = check_level_codec(f);
   if (rc != HC_OK) return rc;
   const int nch = out_channels(f);
@@ -1515,7 +1529,33 @@ int hc_wsi_read_region(const hc_wsi* h, int level, int64_t x, int64_t y, int64_t
   if (!h || !out) return fail(HC_ERR_VALUE, "null argument");
   if (level < 0 || level >= (int)h->w.levels.size()) return fail(HC_ERR_VALUE, "level out of range");
   if (w <= 0 || hgt <= 0) return fail(HC_ERR_VALUE, "region size must be positive");
-  return read_region_host(h->w, level, x, y, w, hgt, out, num_workers);
+  return read_region_host(h->w, h->w.ifds[h->w.levels[level]], x, y, w, hgt, out, num_workers);
+}
+
+int hc_wsi_associated_count(const hc_wsi* h) { return h ? (int)h->w.assoc.size() : -1; }
+
+int hc_wsi_associated_info(const hc_wsi* h, int k, char* name, int name_len, int64_t* info) {
+  if (!h || !info) return fail(HC_ERR_VALUE, "null argument");
+  if (k < 0 || k >= (int)h->w.assoc.size()) return fail(HC_ERR_VALUE, "associated image out of range");
+  const auto& a = h->w.assoc[k];
+  const Ifd& f = h->w.ifds[a.second];
+  if (name && name_len > 0) {
+    const size_t n = std::min<size_t>((size_t)name_len - 1, a.first.size());
+    std::memcpy(name, a.first.data(), n);
+    name[n] = 0;
+  }
+  info[0] = f.width;
+  info[1] = f.height;
+  info[2] = out_channels(f);
+  info[3] = f.compression;
+  return HC_OK;
+}
+
+int hc_wsi_associated_read(const hc_wsi* h, int k, uint8_t* out, int num_workers) {
+  if (!h || !out) return fail(HC_ERR_VALUE, "null argument");
+  if (k < 0 || k >= (int)h->w.assoc.size()) return fail(HC_ERR_VALUE, "associated image out of range");
+  const Ifd& f = h->w.ifds[h->w.assoc[k].second];
+  return read_region_host(h->w, f, 0, 0, f.width, f.height, out, num_workers);
 }
 
 int hc_wsi_read_region_device(const hc_wsi* h, int level, int64_t x, int64_t y, int64_t w,
@@ -1541,7 +1581,7 @@ int hc_wsi_read_region_device(const hc_wsi* h, int level, int64_t x, int64_t y, 
       HC_HIP(hipHostMalloc((void**)&h->pinned, (size_t)nbytes));
       h->pinned_cap = (size_t)nbytes;
     }
-    rc = read_region_host(ws, level, x, y, w, hgt, h->pinned, num_workers);
+    rc = read_region_host(ws, f, x, y, w, hgt, h->pinned, num_workers);
     if (rc == HC_OK) {
       hipError_t e = hipMemcpyAsync(out_dev, h->pinned, (size_t)nbytes, hipMemcpyHostToDevice, st);
       if (e == hipSuccess) e = hipStreamSynchronize(st);

@@ -493,6 +493,15 @@ int hc_wsi_properties(const hc_wsi* h, double* res, int* background, char* desc,
  * calling thread). */
 int hc_wsi_read_region(const hc_wsi* h, int level, int64_t x, int64_t y, int64_t w, int64_t hgt,
                        uint8_t* out, int num_workers);
+/* Associated images of Aperio files (thumbnail, label, macro: untiled IFDs
+ * named by their position and subfile type, tiff.cpp:670-700;
+ * CuImage::associated_images / associated_image, cuimage.h).  info[4]: width,
+ * height, output samples per pixel, compression.  read: the whole image as
+ * height x width x samples uint8 into host memory (strips decoded on
+ * num_workers host threads). */
+int hc_wsi_associated_count(const hc_wsi* h);
+int hc_wsi_associated_info(const hc_wsi* h, int k, char* name, int name_len, int64_t* info);
+int hc_wsi_associated_read(const hc_wsi* h, int k, uint8_t* out, int num_workers);
 /* The same region into device memory: JPEG tiles are uploaded compressed
  * and decoded on the GPU (entropy decoding one work-group per tile from LDS,
  * integer inverse DCT, fancy chroma upsampling and YCbCr -> RGB in device

@@ -123,6 +123,17 @@ class TiffFile:
               if i == 0 or self.vendor == "generic" or f.tiled]
         lv.sort(key=lambda i: (-self.ifds[i].width, -self.ifds[i].height))
         self.levels = lv
+        # associated images of Aperio files (tiff.cpp:670-700)
+        self.associated = {}
+        if self.vendor == "aperio":
+            for i, f in enumerate(self.ifds):
+                if i == 0 or f.tiled:
+                    continue
+                sub = f.tags.get(254, (0,))[0]
+                name = ("thumbnail" if i == 1 and sub == 0 else
+                        "label" if sub == 1 else "macro" if sub == 9 else None)
+                if name:
+                    self.associated[name] = i
         f0 = self.ifds[lv[0]]
         self.downsamples = []
         for i in lv:
@@ -239,7 +250,11 @@ class TiffFile:
 
     def level_image(self, level):
         """The whole level as (height, width, channels) uint8."""
-        f = self.ifds[self.levels[level]]
+        return self.ifd_image(self.levels[level])
+
+    def ifd_image(self, index):
+        """The whole image of IFD ``index`` as (height, width, channels) uint8."""
+        f = self.ifds[index]
         nch = self.out_channels(f)
         img = np.zeros((f.height, f.width, nch), np.uint8)
         ntx = -(-f.width // f.tw)

@@ -200,6 +200,37 @@ def test_aperio_background_and_associated(tmp_path):
     assert _cuimage(p2).resolutions["level_dimensions"] == ((180, 200), (90, 100))
 
 
+@pytest.mark.parametrize("comp", ["lzw", "jpeg", "deflate"])
+def test_associated_images(tmp_path, comp):
+    """Aperio associated images: the untiled IFD 1 of subfile type 0 is the
+    thumbnail, subfile 1 the label, subfile 9 the macro; each read whole
+    equals the oracle's decode (the reference reads JPEG and LZW strips,
+    tiff.cpp:790-900); generic files have none."""
+    lv = pyramid(200, 180, 1, seed=8)
+    rng = np.random.default_rng(9)
+    thumb = pyramid(50, 45, 1, seed=10)[0]
+    label = rng.integers(0, 256, (40, 60, 3)).astype(np.uint8)
+    macro = pyramid(37, 70, 1, seed=11)[0]
+    p = write_tiff(tmp_path / "a.svs", lv, tile=(64, 64), compression=comp,
+                   **({"photometric": "rgb", "subsampling": 0} if comp == "jpeg" else {}),
+                   description=b"Aperio Image Library v10.0.50",
+                   associated=[(thumb, 0), (label, 1), (macro, 9)])
+    im, o = _cuimage(p), TiffFile(p)
+    assert im.associated_images == {"thumbnail", "label", "macro"} == set(o.associated)
+    assert im.resolutions["level_count"] == 1
+    for name, src in [("thumbnail", thumb), ("label", label), ("macro", macro)]:
+        a = im.associated_image(name, num_workers=2)
+        e = o.ifd_image(o.associated[name])
+        assert a.shape == src.shape and np.array_equal(a, e), name
+        if comp != "jpeg":
+            assert np.array_equal(a, src)
+    with pytest.raises(ValueError, match="no associated image"):
+        im.associated_image("barcode")
+    g = write_tiff(tmp_path / "g.tif", lv, tile=(64, 64), compression=comp,
+                   associated=[(label, 1)])
+    assert _cuimage(g).associated_images == set()
+
+
 def test_argument_errors(tmp_path):
     lv = pyramid(64, 48, 1)
     p = write_tiff(tmp_path / "e.tif", lv, tile=(16, 16), compression="raw")
@@ -323,6 +354,20 @@ def test_device_region_matches_oracle(gpu, case):
             assert got.is_cuda and got.dtype == torch.uint8
             exp = o.read_region(loc, size, level)
             assert np.array_equal(got.cpu().numpy(), exp), (name, level, loc, size)
+
+
+@pytest.mark.gpu
+def test_device_associated_image(gpu, tmp_path):
+    import torch
+
+    lv = pyramid(128, 128, 1, seed=12)
+    label = pyramid(40, 56, 1, seed=13)[0]
+    p = write_tiff(tmp_path / "a.svs", lv, tile=(64, 64), compression="jpeg",
+                   description=b"Aperio Image Library", associated=[(label, 1)])
+    im = _cuimage(p)
+    d = im.associated_image("label", device="cuda")
+    assert d.is_cuda and d.dtype == torch.uint8
+    assert np.array_equal(d.cpu().numpy(), im.associated_image("label"))
 
 
 @pytest.mark.gpu
