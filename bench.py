@@ -94,6 +94,9 @@ def per_kernel_stats(records):
     return {k: (len(v), sum(v) / len(v), sum(v)) for k, v in agg.items()}
 
 
+DROPIN = {}
+
+
 def bench_frangi(steps, warmup, check=True):
     import numpy as np
     import torch
@@ -119,6 +122,17 @@ def bench_frangi(steps, warmup, check=True):
             step(i)
         torch.cuda.synchronize()
     stats = per_kernel_stats(kt.records)
+    # the drop-in call (a plan built per call, as frangi() does) for its wall
+    # time next to the plan-reusing step
+    from cucim_amd.skimage.filters import frangi
+
+    frangi(vol, sigmas=SIGMAS, black_ridges=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        frangi(vol, sigmas=SIGMAS, black_ridges=False)
+    torch.cuda.synchronize()
+    DROPIN["frangi_wall_ms_per_call"] = (time.perf_counter() - t0) / 5 * 1e3
     if not check:
         return vol, plan, ms, stats, None, None
     # fused schedule vs the reference's pass order, both on the GPU
@@ -855,6 +869,7 @@ def main():
         "frangi_flops_per_voxel": flops_vox,
         "frangi_tflops": flops_vox * nvox / (ms * 1e-3) / 1e12,
         "fused_vs_reference_order": {"rel_l2": rel_fused, "max_abs": max_fused},
+        "dropin_frangi_wall_ms_per_call": DROPIN.get("frangi_wall_ms_per_call"),
         "correlate1d": corr,
         "cpu_baseline": cpu,
         "parity": parity,

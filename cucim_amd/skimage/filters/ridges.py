@@ -2,6 +2,7 @@
 and the hybrid Hessian filter (``filters/ridges.py`` of the reference, paths
 relative to ``python/cucim/src/cucim/skimage/``)."""
 
+import functools
 from warnings import warn
 
 import numpy as np
@@ -39,10 +40,7 @@ class FrangiPlan:
         self.work = [_array.empty(shape, self.dtype) for _ in range(nH)]
         self.work_hc = _array.hc_arrays(self.work)
         self.scratch = torch.zeros(64, dtype=torch.uint8, device=_array.device())
-        kern = []
-        for s in self.sigmas:
-            kern += hessian_kernels(s, self.ndim, self.dtype)
-        self.ts, self.td, self._keep = _taps_arrays(kern, self.dtype == np.float64)
+        self.ts, self.td, self._keep = _plan_taps(self.ndim, self.dtype.str, tuple(self.sigmas))
 
     def run(self, image, out, alpha=0.5, beta=0.5, gamma=None, black_ridges=True, cval=0,
             center=None):
@@ -56,6 +54,18 @@ class FrangiPlan:
             0 if gamma is None else 1, 1 if black_ridges else 0, _mode_code(self.mode),
             float(cval), self.work_hc, len(self.work), self.scratch.data_ptr(),
             None if center is None else center.data_ptr(), _array.stream())
+
+
+@functools.lru_cache(maxsize=32)
+def _plan_taps(ndim, dtype_str, sigmas):
+    """The host-built Gaussian (derivative) tap arrays of every sigma and
+    axis, shared by the plans of one (ndim, dtype, sigmas): repeated frangi()
+    calls of one configuration then build them once."""
+    dtype = np.dtype(dtype_str)
+    kern = []
+    for s in sigmas:
+        kern += hessian_kernels(s, ndim, dtype)
+    return _taps_arrays(kern, dtype == np.float64)
 
 
 def _frangi_sequential(image, sigmas, alpha, beta, gamma, black_ridges, mode, cval, out):
