@@ -119,6 +119,9 @@ constexpr int kStrWaveBlocks = kStrTA / 64;  // 16-output blocks per wave
 
 // 16-byte loads each lane keeps in flight while staging.
 constexpr int kVecBatch = 8;
+// 16-byte chunks a lane loads per round when it stages a row segment: a
+// 1024-output segment and its halo are about 5 chunks per lane
+constexpr int kRowBatch = 5;
 
 // 16-byte vectors of the compute type.
 template <typename W> struct Vec16;
@@ -144,40 +147,44 @@ __device__ __forceinline__ bool stage_row_vec(W* __restrict__ ext, int L, int la
   const int64_t a0 = p0 >= 0 ? (p0 / VN) * VN : -((-p0 + VN - 1) / VN) * VN;
   const int shift = (int)(p0 - a0);
   const int nc = (shift + L + VN - 1) / VN;
+  // the row holds a whole number of 16-byte chunks (N a multiple of VN,
+  // N >= VN): gl is the start of its last one
+  const int64_t gl = N - VN;
   bool bad = false;
-  for (int c0 = lane; c0 < nc; c0 += 64 * kVecBatch) {
-    W v[kVecBatch][VN];
+  for (int c0 = lane; c0 < nc; c0 += 64 * kRowBatch) {
+    // every load of the batch is issued before any is used: unconditional,
+    // its chunk index clamped into the row (a chunk that is not wholly
+    // inside is replaced below); a load under a branch would be waited for
+    // before the next one is issued
+    V t[kRowBatch];
 #pragma unroll
-    for (int u = 0; u < kVecBatch; ++u) {
-      const int c = c0 + 64 * u;
-      const int64_t g = a0 + (int64_t)c * VN;
-#pragma unroll
-      for (int j = 0; j < VN; ++j) v[u][j] = cval;
-      if (c < nc) {
-        if (g >= 0 && g + VN <= N) {
-          const V t = *reinterpret_cast<const V*>(row + g);
-#pragma unroll
-          for (int j = 0; j < VN; ++j) v[u][j] = t[j];
-        } else {
-#pragma unroll
-          for (int j = 0; j < VN; ++j) {
-            const int64_t r = remap(g + j, N, mode);
-            if (r >= 0) v[u][j] = row[r];
-          }
-        }
-      }
+    for (int u = 0; u < kRowBatch; ++u) {
+      const int64_t g = a0 + (int64_t)(c0 + 64 * u) * VN;
+      const int64_t gc = g < 0 ? 0 : (g > gl ? gl : g);
+      t[u] = *reinterpret_cast<const V*>(row + gc);
     }
 #pragma unroll
-    for (int u = 0; u < kVecBatch; ++u) {
+    for (int u = 0; u < kRowBatch; ++u) {
       const int c = c0 + 64 * u;
-      if (c < nc) {
+      if (c >= nc) continue;
+      const int64_t g = a0 + (int64_t)c * VN;
+      W v[VN];
+      if (g >= 0 && g + VN <= N) {
+#pragma unroll
+        for (int j = 0; j < VN; ++j) v[j] = t[u][j];
+      } else {
 #pragma unroll
         for (int j = 0; j < VN; ++j) {
-          const int sl = c * VN + j - shift;
-          if (sl >= 0 && sl < L) {
-            ext[sl] = v[u][j];
-            bad |= !finite(v[u][j]);
-          }
+          const int64_t r = remap(g + j, N, mode);
+          v[j] = r >= 0 ? row[r] : cval;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < VN; ++j) {
+        const int sl = c * VN + j - shift;
+        if (sl >= 0 && sl < L) {
+          ext[sl] = v[j];
+          bad |= !finite(v[j]);
         }
       }
     }
