@@ -1092,7 +1092,52 @@ __device__ inline int block_excl_scan(int v, int* sh, int* total) {
   return off + incl - v;
 }
 
-__global__ __launch_bounds__(kJLanes) void k_jpeg_entropy_lds(const JDev a, int cap) {
+// Restart intervals (DRI) decode independently: each starts byte-aligned
+// after its RST marker with the DC predictors at zero.  Thread tid decodes
+// intervals tid, tid + kJLanes, ... of R MCUs each; ist[k] is the destuffed
+// byte position where interval k + 1 starts (nmk markers found).
+__device__ void jseg_intervals(const JSeg& sg, const int* ist, int nmk, int nint, int R,
+                               int tid) {
+  const int nmcu = sg.total / sg.U;
+  for (int i = tid; i < nint; i += kJLanes) {
+    const int st = i == 0 ? 0 : (i - 1 < nmk ? ist[i - 1] : sg.n);
+    LdsBits br{sg.s, sg.n, 0, 0, 0ull};
+    br.pos = (st >> 2) << 2;
+    br.refill();
+    br.refill();
+    br.skip((st & 3) * 8);
+    int p0 = 0, p1 = 0, p2 = 0;
+    const int m1 = min((i + 1) * R, nmcu);
+    for (int m = i * R; m < m1; ++m)
+      for (int u = 0; u < sg.U; ++u) {
+        const int c = sg.ucomp[u];
+        int16_t* blk = jseg_block(sg, m * sg.U + u, u);
+        int sdc = huff_lds(br, sg.tabs[sg.dtab[c]]);
+        if (sdc > 15) sdc = 0;
+        const int diff = extend(br.get(sdc), sdc);
+        int& pr = c == 0 ? p0 : (c == 1 ? p1 : p2);
+        pr += diff;
+        blk[0] = (int16_t)pr;
+        const HuffTab& act = sg.tabs[sg.atab[c]];
+        for (int z = 1; z < 64;) {
+          const int rs = huff_lds(br, act);
+          const int r = rs >> 4, sz = rs & 15;
+          if (sz == 0) {
+            if (r != 15) break;
+            z += 16;
+            continue;
+          }
+          z += r;
+          blk[sg.zz[z < 63 ? z : 63]] = (int16_t)extend(br.get(sz), sz);
+          ++z;
+        }
+      }
+  }
+}
+
+// max_iv: entries of the restart-interval table after the staged segment in
+// dynamic LDS (0: no tile of the launch has restart markers).
+__global__ __launch_bounds__(kJLanes) void k_jpeg_entropy_lds(const JDev a, int cap, int max_iv) {
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ int scan_sh[kJWaves];
   __shared__ int64_t end_bit[kJLanes];
@@ -1102,10 +1147,13 @@ __global__ __launch_bounds__(kJLanes) void k_jpeg_entropy_lds(const JDev a, int 
   const JTile& jt = a.tiles[t];
   int16_t* coef = a.coef + jt.coef0 * 64;
   const int64_t n = jt.data_len;
-  if (jt.restart != 0 || n > cap || n < 256) {
+  const int R = jt.restart;
+  const int nint = R > 0 ? (jt.mcux * jt.mcuy + R - 1) / R : 0;
+  if (nint > max_iv || n > cap || n < 256) {
     if (tid == 0) jpeg_entropy(jt, a.tabsets[jt.tabset], a.data, coef);
     return;
   }
+  int* ist = reinterpret_cast<int*>(smem + kTabBytes + cap + 16);
   // tables: dc[0], dc[1], ac[0], ac[1] as 32-bit words
   HuffTab* tabs = reinterpret_cast<HuffTab*>(smem);
   {
@@ -1119,28 +1167,46 @@ __global__ __launch_bounds__(kJLanes) void k_jpeg_entropy_lds(const JDev a, int 
       dst[half + i] = src1[i];
     }
   }
-  // destuff the segment into LDS, 16 bytes per thread per round
+  // destuff the segment into LDS, 16 bytes per thread per round; with
+  // restart markers, also drop every RST marker (0xFF 0xD0..0xD7) and record
+  // where the next interval starts
   uint8_t* ds = smem + kTabBytes;
   const uint8_t* g = a.data + jt.data_off;
-  int out = 0;
+  int out = 0, nmk = 0;
+  auto is_rst = [](uint8_t v) { return v >= 0xD0 && v <= 0xD7; };
   for (int64_t base = 0; base < n; base += kJLanes * 16) {
     const int64_t i0 = base + tid * 16;
     uint8_t b[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) b[k] = i0 + k < n ? g[i0 + k] : 0;
     const uint8_t before = (i0 > 0 && i0 - 1 < n) ? g[i0 - 1] : 0;
-    uint32_t keep = 0;
+    const uint8_t after = (R > 0 && i0 + 16 < n) ? g[i0 + 16] : 0;
+    uint32_t keep = 0, mk = 0;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const uint8_t pv = k ? b[k - 1] : before;
-      if (i0 + k < n && !(pv == 0xFF && b[k] == 0x00)) keep |= 1u << k;
+      const uint8_t nx = k < 15 ? b[k + 1] : after;
+      const bool in = i0 + k < n;
+      const bool rst_ff = R > 0 && b[k] == 0xFF && is_rst(nx);
+      const bool rst_n = R > 0 && pv == 0xFF && is_rst(b[k]);
+      if (in && !(pv == 0xFF && b[k] == 0x00) && !rst_ff && !rst_n) keep |= 1u << k;
+      if (in && rst_ff && i0 + k + 1 < n) mk |= 1u << k;
     }
+    // one scan for both counts: kept bytes (< 2^16 per round) and markers
     int total;
-    int pos = out + block_excl_scan(__popc(keep), scan_sh, &total);
+    const int ex = block_excl_scan(__popc(keep) | (__popc(mk) << 16), scan_sh, &total);
+    int pos = out + (ex & 0xFFFF);
+    int mi = nmk + (ex >> 16);
 #pragma unroll
-    for (int k = 0; k < 16; ++k)
+    for (int k = 0; k < 16; ++k) {
+      if (mk & (1u << k)) {
+        if (mi < max_iv) ist[mi] = pos;  // the next kept byte starts an interval
+        ++mi;
+      }
       if (keep & (1u << k)) ds[pos++] = b[k];
-    out += total;
+    }
+    out += total & 0xFFFF;
+    nmk += total >> 16;
   }
   __shared__ uint8_t zz[64];
   __shared__ int8_t ucomp[12], ubx[12], uby[12];
@@ -1180,6 +1246,10 @@ __global__ __launch_bounds__(kJLanes) void k_jpeg_entropy_lds(const JDev a, int 
     seg.cv[c] = jt.comp[cc].v;
     seg.cbw[c] = jt.comp[cc].bw;
     seg.cb0[c] = jt.comp[cc].block0;
+  }
+  if (R > 0) {
+    jseg_intervals(seg, ist, min(nmk, max_iv), nint, R, tid);
+    return;
   }
   // each thread owns S bits (a multiple of 32) of the destuffed segment
   const int64_t nbits = (int64_t)out * 8;
@@ -1719,8 +1789,16 @@ int hc_wsi_read_region_device(const hc_wsi* h, int level, int64_t x, int64_t y, 
   // ones are decoded from global memory)
   constexpr int64_t kMaxStage = 120 * 1024;
   int64_t need = 0;
-  for (int i = 0; i < nt; ++i)
-    if (T[i].restart == 0 && T[i].data_len <= kMaxStage) need = std::max(need, T[i].data_len);
+  int max_iv = 0;  // restart intervals of the largest such tile (table in LDS)
+  constexpr int kMaxIntervals = 4096;  // with a 120 KiB stage: < 160 KiB of LDS
+  for (int i = 0; i < nt; ++i) {
+    if (T[i].data_len > kMaxStage) continue;
+    need = std::max(need, T[i].data_len);
+    if (T[i].restart > 0) {
+      const int nint = (T[i].mcux * T[i].mcuy + T[i].restart - 1) / T[i].restart;
+      if (nint <= kMaxIntervals) max_iv = std::max(max_iv, nint);
+    }
+  }
   const int cap = (int)((need + 16 * kJLanes - 1) / (16 * kJLanes) * (16 * kJLanes));
   // device work buffer: the staged bytes at the same offsets, then the
   // coefficients and the component planes
@@ -1768,7 +1846,7 @@ int hc_wsi_read_region_device(const hc_wsi* h, int level, int64_t x, int64_t y, 
     HC_LAUNCH("k_fill_u32", st, k_fill_u32,
               dim3((unsigned)std::min<int64_t>(4096, (nblk * 32 + 255) / 256)), dim3(256), 0,
               reinterpret_cast<uint32_t*>(a.coef), nblk * 32, 0u);
-    const size_t smem = (size_t)(kTabBytes + cap + 16);
+    const size_t smem = (size_t)(kTabBytes + cap + 16) + 4 * (size_t)max_iv;
     if (smem > 64 * 1024) {
       const hipError_t fe = hipFuncSetAttribute((const void*)k_jpeg_entropy_lds,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1776,7 +1854,7 @@ int hc_wsi_read_region_device(const hc_wsi* h, int level, int64_t x, int64_t y, 
       if (fe != hipSuccess) return hip_check(fe, "hipFuncSetAttribute(k_jpeg_entropy_lds)");
     }
     HC_LAUNCH("k_jpeg_entropy_lds", st, k_jpeg_entropy_lds, dim3((unsigned)nt), dim3(kJLanes),
-              smem, a, cap);
+              smem, a, cap, max_iv);
     HC_LAUNCH("k_jpeg_idct", st, k_jpeg_idct,
               dim3((unsigned)std::min<int64_t>(65535, (nblk + 63) / 64)), dim3(64), 0, a, nblk);
     HC_LAUNCH("k_jpeg_color", st, k_jpeg_color, dim3(64, (unsigned)nt), dim3(256), 0, a);

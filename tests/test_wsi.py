@@ -357,6 +357,22 @@ def test_device_region_matches_oracle(gpu, case):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("restart,sub", [(1, 2), (3, 0), (5, 1), (64, 2)])
+def test_device_restart_intervals(gpu, tmp_path, restart, sub):
+    """JPEG tiles with restart markers (DRI every `restart` MCUs): decoded on
+    the device interval by interval (each starts byte-aligned, DC predictors
+    at zero), equal to the oracle's libjpeg-turbo decode."""
+    lv = pyramid(512, 384, 1, seed=14)
+    p = write_tiff(tmp_path / "r.tif", lv, tile=(128, 128), compression="jpeg",
+                   restart=restart, subsampling=sub, quality=92)
+    o, im = TiffFile(p), _cuimage(p)
+    got = im.read_region((0, 0), (384, 512), 0, device="cuda", num_workers=2)
+    assert np.array_equal(got.cpu().numpy(), o.read_region((0, 0), (384, 512), 0))
+    got = im.read_region((50, 70), (200, 300), 0, device="cuda")
+    assert np.array_equal(got.cpu().numpy(), o.read_region((50, 70), (200, 300), 0))
+
+
+@pytest.mark.gpu
 def test_device_associated_image(gpu, tmp_path):
     import torch
 
