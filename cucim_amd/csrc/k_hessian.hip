@@ -289,8 +289,8 @@ __host__ __device__ constexpr int mfma_npf(int NBW) { return 4 * NBW; }
 // Waves per SIMD the compiler budgets registers for: four blocks per wave
 // need the larger share (the chain pass's two first-op results, the final
 // pass's six Hessian inputs); two or one leave room for more waves.
-__host__ __device__ constexpr int chain_wpe(int NBW) { return NBW >= 4 ? 5 : 6; }
-__host__ __device__ constexpr int final_wpe(int NBW) { return NBW >= 4 ? 4 : 6; }
+__host__ __device__ constexpr int chain_wpe(int NBW) { return NBW >= 3 ? 5 : 6; }
+__host__ __device__ constexpr int final_wpe(int NBW) { return NBW >= 3 ? 4 : 6; }
 
 // Blocks b = wave + NW i (i < NBW) of 16 positions: acc[s][i] for the NS
 // weight sets wp[s] (zero-padded: trimmed tap t at wp[15 + t]).  The A
@@ -1361,6 +1361,7 @@ inline int launch_chain_mfma(ChainArgs<float>& a, hipStream_t st) {
     return nblk <= 16 ? launch_chain_mfma_nbw<2, 8>(a, g, st) : launch_chain_mfma_nbw<4, 8>(a, g, st);
   if (nblk <= 4) return launch_chain_mfma_nbw<1, 4>(a, g, st);
   if (nblk <= 8) return launch_chain_mfma_nbw<2, 4>(a, g, st);
+  if (nblk <= 12) return launch_chain_mfma_nbw<3, 4>(a, g, st);
   return launch_chain_mfma_nbw<4, 4>(a, g, st);
 }
 
@@ -1480,6 +1481,7 @@ int launch_final_mfma(const FinalArgs<float>& a0, hipStream_t st) {
                       : launch_final_mfma_nbw<NIN, 4, 8>(a, g, st);
   if (nblk <= 4) return launch_final_mfma_nbw<NIN, 1, 4>(a, g, st);
   if (nblk <= 8) return launch_final_mfma_nbw<NIN, 2, 4>(a, g, st);
+  if (nblk <= 12) return launch_final_mfma_nbw<NIN, 3, 4>(a, g, st);
   return launch_final_mfma_nbw<NIN, 4, 4>(a, g, st);
 }
 
