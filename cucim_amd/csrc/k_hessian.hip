@@ -1360,6 +1360,7 @@ inline int launch_chain_mfma(ChainArgs<float>& a, hipStream_t st) {
   if (mfma_waves(a.N) == 8)
     return nblk <= 16 ? launch_chain_mfma_nbw<2, 8>(a, g, st) : launch_chain_mfma_nbw<4, 8>(a, g, st);
   if (nblk <= 4) return launch_chain_mfma_nbw<1, 4>(a, g, st);
+  if (nblk <= 6) return launch_chain_mfma_nbw<3, 2>(a, g, st);
   if (nblk <= 8) return launch_chain_mfma_nbw<2, 4>(a, g, st);
   if (nblk <= 12) return launch_chain_mfma_nbw<3, 4>(a, g, st);
   return launch_chain_mfma_nbw<4, 4>(a, g, st);
@@ -1480,6 +1481,7 @@ int launch_final_mfma(const FinalArgs<float>& a0, hipStream_t st) {
     return nblk <= 16 ? launch_final_mfma_nbw<NIN, 2, 8>(a, g, st)
                       : launch_final_mfma_nbw<NIN, 4, 8>(a, g, st);
   if (nblk <= 4) return launch_final_mfma_nbw<NIN, 1, 4>(a, g, st);
+  if (nblk <= 6) return launch_final_mfma_nbw<NIN, 3, 2>(a, g, st);
   if (nblk <= 8) return launch_final_mfma_nbw<NIN, 2, 4>(a, g, st);
   if (nblk <= 12) return launch_final_mfma_nbw<NIN, 3, 4>(a, g, st);
   return launch_final_mfma_nbw<NIN, 4, 4>(a, g, st);

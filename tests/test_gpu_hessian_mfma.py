@@ -74,7 +74,8 @@ def test_mfma_hessian_vs_fp64_and_valu(gpu, shape, sigma, mode):
 
 
 @pytest.mark.parametrize("shape", [(40, 33), (5, 300), (3, 17, 260), (21, 9, 7),
-                                   (140, 6, 150), (9, 190, 177), (133, 20)])
+                                   (140, 6, 150), (9, 190, 177), (133, 20), (70, 5, 90),
+                                   (81, 96)])
 @pytest.mark.parametrize("sigma", [1.0, 3.0, 5.0])
 @pytest.mark.parametrize("mode", ["nearest", "mirror"])
 def test_mfma_hessian_other_modes(gpu, shape, sigma, mode):
