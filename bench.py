@@ -725,13 +725,13 @@ def bench_wsi(reps=5, cpu=True):
         t = timed(pipe, reps)
         res["read_gray_gaussian_ms"] = t * 1e3
         res["host_threads_for_tile_reads"] = nw
-        nw = min(16, os.cpu_count() or 1)
+        nwh = min(16, os.cpu_count() or 1)
         hb = np.empty((edge, edge, 3), np.uint8)
-        im.read_region((0, 0), (edge, edge), 0, num_workers=nw, buf=hb)
+        im.read_region((0, 0), (edge, edge), 0, num_workers=nwh, buf=hb)
         t0 = time.perf_counter()
-        im.read_region((0, 0), (edge, edge), 0, num_workers=nw, buf=hb)
+        im.read_region((0, 0), (edge, edge), 0, num_workers=nwh, buf=hb)
         th = time.perf_counter() - t0
-        res["host_read"] = {"ms": th * 1e3, "mpx_s": edge * edge / th / 1e6, "threads": nw}
+        res["host_read"] = {"ms": th * 1e3, "mpx_s": edge * edge / th / 1e6, "threads": nwh}
         res["device_equals_host"] = bool(np.array_equal(out.cpu().numpy(), hb))
         if cpu:
             from oracle.wsi import TiffFile
