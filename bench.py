@@ -744,6 +744,21 @@ def bench_wsi(reps=5, cpu=True):
                                    "cores": 1, "kind": "port",
                                    "sample": "every level-0 tile decoded by libjpeg-turbo "
                                              "(Pillow) and assembled, one thread"}
+            # the same tile decodes on every host thread (Pillow releases the
+            # GIL while libjpeg-turbo decodes)
+            from concurrent.futures import ThreadPoolExecutor
+
+            f0 = o.ifds[o.levels[0]]
+            with ThreadPoolExecutor(nwh) as ex:
+                list(ex.map(lambda i: o.decode_tile(f0, i), range(len(f0.offsets))))
+                t0 = time.perf_counter()
+                list(ex.map(lambda i: o.decode_tile(f0, i), range(len(f0.offsets))))
+                tm = time.perf_counter() - t0
+            res["cpu_baseline_all_threads"] = {"value": edge * edge / tm / 1e6, "unit": "Mpx/s",
+                                               "cores": nwh, "kind": "port",
+                                               "sample": "every level-0 tile decoded by "
+                                                         "libjpeg-turbo (Pillow) on a thread "
+                                                         "pool, not assembled"}
         im.close()
     return res
 
